@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark of the chunk pipeline (noise -> fill -> cull/emit) on one MI355X.
+
+Workload (the metric's configuration): a uniform batch of 4096 chunks of
+32^3 voxels on a 64 x 64 grid at y = 0; 3-D fBm Perlin density (seed 7,
+4 octaves, lacunarity 2, gain 0.5, base frequency 1/64), solid where
+density - y/128 > 0; every in-grid neighbour present.  Outputs per step:
+u8 block ids, packed u32 occupancy rows, boundary planes, and the mesh as
+4 vertices (f32 xyz+uv) + 6 u32 indices per visible face.  A step is one
+replay of the captured graph fill -> count -> scan -> emit over the batch;
+inputs (the request table) are resident in HBM before timing starts.
+
+Prints ONE JSON line on stdout.  Progress goes to stderr.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "wgpu-voxel-engine_amd"))
+
+import numpy as np  # noqa: E402
+
+import voxel_amd as V  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3    # MI355X published fp32 peak
+GRID = 64                   # 64 x 64 chunks
+EDGE = 32
+# FP32 operations of one restated 3-D Improved Noise evaluation inside fBm
+# (oracle/voxel_oracle.c noise3f + vxo_fbm3): 3 scale, 3 floor, 3 frac,
+# 3 fades x 7, 3 "-1", 8 gradient adds, 7 lerps x 3, 2 accumulate = 64.
+K_EVAL = 64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(ctx):
+    t = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
+                  gain=0.5, y_scale=1 / 128)
+    coords = np.array([(x, 0, z) for x in range(GRID) for z in range(GRID)], np.int32)
+    return t, coords, V.grid_neighbours(GRID, GRID)
+
+
+def cpu_baseline(target_s: float = 12.0):
+    """The oracle's multithreaded driver (one chunk per task, all host
+    cores) on a bounded sub-grid of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.build()
+    threads = len(os.sched_getaffinity(0))
+    t = O.terrain(kind=3, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0, gain=0.5,
+                  y_scale=1 / 128)
+    # warm-up and sizing on an 8x8 sub-grid
+    side = 8
+    coords = np.array([(x, 0, z) for x in range(side) for z in range(side)], np.int32)
+    t0 = time.perf_counter()
+    O.batch_run(t, coords, V.grid_neighbours(side, side), EDGE, threads)
+    per_chunk = (time.perf_counter() - t0) / len(coords)
+    reps = 3
+    n_target = max(64, int(target_s / reps / max(per_chunk, 1e-6)))
+    side = int(min(GRID, max(8, int(np.sqrt(n_target)))))
+    coords = np.array([(x, 0, z) for x in range(side) for z in range(side)], np.int32)
+    nb = V.grid_neighbours(side, side)
+    rates = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.batch_run(t, coords, nb, EDGE, threads)
+        rates.append(len(coords) / (time.perf_counter() - t0))
+    cpu_model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(statistics.median(rates), 1), "unit": "chunks/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{side}x{side} sub-grid ({side * side} chunks) of the same 32^3 workload, "
+                      f"fill (AVX2 noise) + scalar cull/emit to vertices+indices, median of "
+                      f"{reps} runs on {threads} threads ({cpu_model})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-chunk GPU path")
+    ap.add_argument("--profile-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(chunks shard by spatial tile; see the design notes). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ctx = V.Context(0)
+    t, coords, nb = workload(ctx)
+    tid = ctx.terrain(t)
+    n = len(coords)
+    log(f"[bench] creating batch: {n} chunks of {EDGE}^3")
+    b = V.Batch(ctx, coords, EDGE, tid, nb, formats=V.FMT_VERTICES)
+    b.run()
+    total_faces = b.wait()           # sizes the mesh arena (grows once if needed)
+    log(f"[bench] faces per step: {total_faces}")
+    for _ in range(args.warmup):
+        b.run()
+    ctx.synchronize()
+    # timed region: K graph replays bracketed by device synchronisation
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert b.wait() == total_faces
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n * args.steps / elapsed
+    log(f"[bench] {ms_per_step:.3f} ms/step, {value:.0f} chunks/s")
+
+    # per-kernel HIP-event timing on the context stream
+    stage_ms = b.profile(args.profile_reps)
+    names = ["fill", "count", "scan", "emit"]
+    log("[bench] stage ms: " + ", ".join(f"{k}={v:.4f}" for k, v in zip(names, stage_ms[:4])) +
+        f", graph={stage_ms[4]:.4f}")
+    N3, N2 = EDGE ** 3, EDGE ** 2
+    fill_bytes = n * (N3 + N3 // 8 + 4 * N2 // 8 + 8)          # ids + occupancy + planes + meta
+    fill_flops = n * N3 * t.octaves * K_EVAL
+    emit_bytes = n * (N3 // 8 + 4 * N2 // 8 + 4 + 8) + 104 * total_faces
+    stages = {
+        "fill": {"ms": round(float(stage_ms[0]), 4),
+                 "GB/s": round(fill_bytes / (stage_ms[0] * 1e-3) / 1e9, 1),
+                 "TFLOP/s": round(fill_flops / (stage_ms[0] * 1e-3) / 1e12, 2),
+                 "algorithmic_bytes": fill_bytes, "algorithmic_flops": fill_flops},
+        "count": {"ms": round(float(stage_ms[1]), 4)},
+        "scan": {"ms": round(float(stage_ms[2]), 4)},
+        "emit": {"ms": round(float(stage_ms[3]), 4),
+                 "GB/s": round(emit_bytes / (stage_ms[3] * 1e-3) / 1e9, 1),
+                 "algorithmic_bytes": emit_bytes},
+    }
+    if stage_ms[0] >= stage_ms[3]:
+        ach = fill_flops / (stage_ms[0] * 1e-3) / 1e12
+        roof = {"kernel": "fill_kernel<32> (noise+fill)", "bound": "valu", "achieved": round(ach, 2),
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
+                "traffic": None}
+    else:
+        ach = emit_bytes / (stage_ms[3] * 1e-3) / 1e9
+        roof = {"kernel": "emit_kernel<32> (cull+compact+emit)", "bound": "hbm",
+                "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+
+    plain = None
+    if not args.no_plain:
+        b.run_per_chunk()
+        ctx.synchronize()
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run_per_chunk()
+        ctx.synchronize()
+        pe = time.perf_counter() - t0
+        assert b.wait() == total_faces
+        plain = {"value": round(n * reps / pe, 1), "unit": "chunks/s",
+                 "ms_per_step": round(pe * 1e3 / reps, 3),
+                 "launches_per_step": 3 * n + 1}
+        log(f"[bench] plain per-chunk path: {plain['value']:.0f} chunks/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle, all host cores)...")
+        cpu = cpu_baseline()
+        log(f"[bench] CPU: {cpu['value']} chunks/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "chunks/sec (noise+mesh, 32^3, bit-exact)",
+        "value": round(value, 1),
+        "unit": "chunks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u8",
+        "data": "synthetic (seeded fBm terrain)",
+        "config": {"workload": "uniform batch: 4096 chunks of 32^3 on a 64x64 grid at y=0, "
+                               "3-D fBm density seed 7 (4 octaves, lac 2, gain 0.5, f0 1/64), "
+                               "solid iff density - y/128 > 0, in-grid neighbours present; "
+                               "outputs u8 ids + u32 occupancy + vertices/indices",
+                   "chunks_per_step": n, "edge": EDGE, "faces_per_step": total_faces},
+        "roofline": roof,
+        "stages": stages,
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+    }
+    if cpu:
+        out["vs_cpu"] = round(value / cpu["value"], 1)
+    if plain:
+        out["vs_plain_gpu"] = round(value / plain["value"], 2)
+    b.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/*
+ * voxel.h -- C ABI of libvoxel_hip.so, the MI355X (gfx950) chunk pipeline:
+ * noise terrain fill -> visible-face culling and quad emission -> AABB
+ * frustum tests, with chunk state and meshes resident in HBM.
+ *
+ * The entry points mirror the module surface the reference's scheduler uses
+ * (paths relative to the reference repository, Blatko1/wgpu-voxel-engine):
+ *
+ *   vx_chunk_new            Chunk::new                 src/chunk.rs:21-25
+ *   vx_chunk_create_mesh    Chunk::create_mesh         src/chunk.rs:48-57
+ *                           + ChunkMesh::new           src/chunk.rs:254-266
+ *   vx_frustum_from_matrix  Frustum::new / update      src/frustum_culling.rs:10-14,42-45
+ *   vx_frustum_contains     Frustum::contains          src/frustum_culling.rs:16-40
+ *   vx_frustum_mask         World::render's per-mesh Frustum::contains loop
+ *                                                      src/world.rs:34-38
+ *   vx_batch_*              many ChunkGenerator jobs at once: the body of
+ *                           pool.execute(Chunk::new + create_mesh)
+ *                                                      src/chunk_builder.rs:80-84
+ *   vx_submit / vx_poll     pool.execute / flume try_recv
+ *                                                      src/chunk_builder.rs:80-86,310-332
+ *   vx_world_*              World + ChunkGenerator     src/world.rs:14-58,
+ *                                                      src/chunk_builder.rs:45-352
+ *
+ * Conventions
+ *   - Every call returns an int status (VX_OK = 0, negative = error); nothing
+ *     panics or aborts across the ABI (the reference unwrap()s:
+ *     src/chunk_builder.rs:83,107,128).
+ *   - Chunks and meshes are immutable after construction and reference
+ *     counted like the reference's Arc<Chunk> (src/chunk_builder.rs:81,140);
+ *     release every handle you own with vx_*_release.
+ *   - Calls on one vx_ctx are serialised by the context; any host thread may
+ *     call them.  GPU work is queued on the context's stream.
+ *   - Coordinates are chunk coordinates (ChunkCoord3D, src/coordinate.rs:41-66);
+ *     a chunk of edge N covers world voxels [c*N, c*N + N) on each axis (the
+ *     reference fixes N = 32, src/coordinate.rs:60-65).
+ *   - Block ids are u8: 0 air, 1 grass, 2 dirt (the reference's CubeType
+ *     GRASS=0, DIRT=1, AIR=2, src/cube.rs:16-21, shifted so 0 is air).
+ *   - Voxel index inside a chunk: x + N*z + N*N*y (src/chunk.rs:38).
+ */
+#ifndef VOXEL_H
+#define VOXEL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VX_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------- */
+#define VX_OK 0
+#define VX_ERR_INVALID (-1)  /* bad argument */
+#define VX_ERR_NOMEM (-2)    /* host or device allocation failed */
+#define VX_ERR_DEVICE (-3)   /* HIP runtime error */
+#define VX_ERR_CAPACITY (-4) /* caller buffer too small; required size reported */
+#define VX_ERR_BUDGET (-5)   /* HBM admission budget exceeded */
+#define VX_ERR_NOT_READY (-6)
+#define VX_ERR_NODEV (-7)    /* no usable GPU */
+
+/* ---- types --------------------------------------------------------------- */
+typedef struct vx_ctx vx_ctx;
+typedef struct vx_chunk vx_chunk;
+typedef struct vx_mesh vx_mesh;
+typedef struct vx_batch vx_batch;
+typedef struct vx_world vx_world;
+
+typedef struct vx_coord {
+    int32_t x, y, z;
+} vx_coord;
+
+/* Terrain generator.  kind:
+ *   VX_TERRAIN_AIR / VX_TERRAIN_SOLID   every voxel air / grass
+ *   VX_TERRAIN_HEIGHTMAP  n = fbm2(wx*f, wz*f); h = floor(height_scale*(n+1)/2)
+ *                         clamped to [0, height_max]; voxel solid iff wy <= h
+ *   VX_TERRAIN_DENSITY    solid iff fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0
+ * fbm: sum over `octaves` of amp*perlin(p*freq), freq *= lacunarity,
+ * amp *= gain, from freq = frequency, amp = 1 (the reference's fbm_3d_offset
+ * with_freq/with_gain/with_lacunarity/with_octaves, src/perlin_noise.rs:6-20).
+ * perlin = Improved Noise (Perlin 2002) with the permutation shuffled by
+ * splitmix64(seed).  Every octave frequency must be >= 2^-20. */
+#define VX_TERRAIN_AIR 0
+#define VX_TERRAIN_SOLID 1
+#define VX_TERRAIN_HEIGHTMAP 2
+#define VX_TERRAIN_DENSITY 3
+
+typedef struct vx_terrain {
+    int32_t kind;
+    uint32_t seed;
+    int32_t octaves; /* 1..8 */
+    float frequency;
+    float lacunarity;
+    float gain;
+    float height_scale;
+    int32_t height_max;
+    float y_scale;
+} vx_terrain;
+
+/* Mesh output formats (bit mask). */
+#define VX_FMT_INSTANCES 1u /* InstanceRaw: f32 4x4 column-major + u32 layer, 68 B
+                               (src/renderer/instance.rs:1-6, src/quad.rs:47-54) */
+#define VX_FMT_VERTICES 2u  /* 4 Vertex {f32 pos[3], f32 uv[2]} (20 B) + 6 u32
+                               chunk-local indices per face: 104 B per face
+                               (src/quad.rs:66-89, src/shaders/vertex.vert:24) */
+
+typedef struct vx_ctx_desc {
+    int32_t device;        /* HIP device ordinal */
+    uint32_t flags;        /* reserved, 0 */
+    uint64_t hbm_budget;   /* bytes the context may hold resident; 0 = no limit */
+} vx_ctx_desc;
+
+/* ---- library / context --------------------------------------------------- */
+int vx_abi_version(void);
+const char *vx_status_string(int status);
+/* Number of visible HIP devices (0 when none); does not create a context. */
+int vx_device_count(int32_t *count);
+
+int vx_ctx_create(const vx_ctx_desc *desc, vx_ctx **out);
+int vx_ctx_destroy(vx_ctx *ctx);
+int vx_ctx_synchronize(vx_ctx *ctx);
+/* The hipStream_t all of this context's work is queued on. */
+int vx_ctx_stream(vx_ctx *ctx, void **hip_stream);
+/* Device bytes currently held by chunks, meshes and batches of the context. */
+int vx_ctx_memory(vx_ctx *ctx, uint64_t *resident_bytes, uint64_t *budget_bytes);
+/* Last HIP error text recorded by the context (static storage). */
+const char *vx_ctx_last_error(vx_ctx *ctx);
+
+/* Registers a terrain generator; returns its id in *terrain_id. */
+int vx_terrain_register(vx_ctx *ctx, const vx_terrain *terrain, int32_t *terrain_id);
+
+/* ---- Chunk (src/chunk.rs:15-46) ------------------------------------------ */
+/* Chunk::new: generates the chunk's block ids on the GPU (asynchronous). */
+int vx_chunk_new(vx_ctx *ctx, int32_t terrain_id, vx_coord coord, int32_t edge,
+                 vx_chunk **out);
+/* Builds a chunk from caller-provided block ids (host memory, edge^3 bytes,
+ * values 0..2).  The reference builds chunks only from noise; this is the
+ * entry point for edited or externally stored chunks. */
+int vx_chunk_from_ids(vx_ctx *ctx, vx_coord coord, int32_t edge, const uint8_t *ids,
+                      vx_chunk **out);
+int vx_chunk_retain(vx_chunk *chunk);
+int vx_chunk_release(vx_chunk *chunk);
+int vx_chunk_info(const vx_chunk *chunk, vx_coord *coord, int32_t *edge);
+/* Copies the block ids (edge^3 bytes) / occupancy rows (edge^2 rows of
+ * (edge+31)/32 u32, bit x of row y*edge+z) to host memory; synchronises. */
+int vx_chunk_read_ids(vx_ctx *ctx, const vx_chunk *chunk, uint8_t *dst, uint64_t cap);
+int vx_chunk_read_occupancy(vx_ctx *ctx, const vx_chunk *chunk, uint32_t *dst,
+                            uint64_t cap_words);
+/* Raw noise the chunk was generated from (density: edge^3 f32 at the voxel
+ * index; heightmap: edge^2 f32 at x + edge*z).  For parity checks. */
+int vx_chunk_noise(vx_ctx *ctx, int32_t terrain_id, vx_coord coord, int32_t edge, float *dst,
+                   uint64_t cap);
+
+/* ---- ChunkMesh (src/chunk.rs:48-57, 248-266) ----------------------------- */
+/* Chunk::create_mesh with the neighbour snapshot adj = {-x, +x, -z, +z}
+ * (src/chunk_builder.rs:134-169); NULL = absent neighbour (its boundary
+ * faces are emitted, src/chunk.rs:102-108).  Neighbours must have the same
+ * edge.  Synchronises once to size the mesh. */
+int vx_chunk_create_mesh(vx_ctx *ctx, vx_chunk *chunk, vx_chunk *const adj[4], uint32_t formats,
+                         vx_mesh **out);
+int vx_mesh_retain(vx_mesh *mesh);
+int vx_mesh_release(vx_mesh *mesh);
+int vx_mesh_face_count(const vx_mesh *mesh, uint64_t *faces);
+/* Copy-out with explicit capacities in bytes; VX_ERR_CAPACITY when short. */
+int vx_mesh_read_instances(vx_ctx *ctx, const vx_mesh *mesh, void *dst, uint64_t cap);
+int vx_mesh_read_vertices(vx_ctx *ctx, const vx_mesh *mesh, void *dst, uint64_t cap);
+int vx_mesh_read_indices(vx_ctx *ctx, const vx_mesh *mesh, void *dst, uint64_t cap);
+/* Zero-copy: device pointers of the mesh's records (NULL for formats not
+ * built).  Valid while the mesh is retained. */
+int vx_mesh_device_buffers(const vx_mesh *mesh, void **instances, void **vertices,
+                           void **indices);
+
+/* ---- Frustum (src/frustum_culling.rs) ------------------------------------ */
+typedef struct vx_frustum {
+    float planes[24]; /* [near far left right top bottom] x (a b c d) */
+} vx_frustum;
+/* matrix_to_planes: m is the view-projection matrix, column-major (m[c*4+r]). */
+int vx_frustum_from_matrix(const float m[16], vx_frustum *out);
+/* contains: 1 if some corner of the chunk AABB is inside all six planes. */
+int vx_frustum_contains(const vx_frustum *f, vx_coord coord, int32_t edge);
+/* The same test for n chunks on the GPU; coords/out are host arrays. */
+int vx_frustum_mask(vx_ctx *ctx, const vx_frustum *f, const vx_coord *coords, int32_t n,
+                    int32_t edge, uint8_t *out);
+/* Device-resident variant: coords_dev (3n i32) and out_dev (n u8) are device
+ * pointers; queued on the context stream, no synchronisation. */
+int vx_frustum_mask_device(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
+                           int32_t n, int32_t edge, uint8_t *out_dev);
+
+/* ---- Batches: many Chunk::new + create_mesh jobs in one launch sequence --- */
+/* One request: chunk coord, edge (16, 32 or 64), terrain, and the neighbour
+ * snapshot as indices into the same batch (-1 = absent). */
+typedef struct vx_batch_req {
+    vx_coord coord;
+    int32_t edge;
+    int32_t terrain_id;
+    int32_t nb[4]; /* -x, +x, -z, +z */
+} vx_batch_req;
+
+int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n, uint32_t formats,
+                    uint64_t face_capacity, vx_batch **out);
+int vx_batch_destroy(vx_batch *batch);
+/* Queues fill -> count -> scan -> emit on the context stream (replayed from a
+ * captured hipGraph after the first run).  Asynchronous. */
+int vx_batch_run(vx_ctx *ctx, vx_batch *batch);
+/* Waits for the last run; if the mesh arena was too small it is grown and the
+ * emit stage re-run.  Reports the batch's total face count. */
+int vx_batch_wait(vx_ctx *ctx, vx_batch *batch, uint64_t *total_faces);
+/* Per-chunk result: first face (batch-global) and face count. */
+int vx_batch_chunk_faces(vx_ctx *ctx, vx_batch *batch, int32_t i, uint64_t *first,
+                         uint64_t *count);
+int vx_batch_read_ids(vx_ctx *ctx, vx_batch *batch, int32_t i, uint8_t *dst, uint64_t cap);
+/* Copies the whole batch's vertex / index / instance buffers (faces in
+ * request order; indices are chunk-local). */
+int vx_batch_read_mesh(vx_ctx *ctx, vx_batch *batch, uint32_t format, void *dst, uint64_t cap);
+int vx_batch_device_buffers(vx_batch *batch, void **instances, void **vertices, void **indices);
+/* Times `reps` replays of each stage with HIP events on the context stream;
+ * ms[0..3] = average fill, count, scan, emit kernel time, ms[4] = whole run. */
+int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
+/* Plain path for comparison: the same work as one launch per chunk per
+ * stage (no batching, no graph), chunk order preserved. */
+int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOXEL_H */

@@ -1,0 +1,281 @@
+"""GPU parity: every output of the HIP path, called through the C ABI, is
+compared with the CPU oracle on the same seeded inputs -- bit for bit
+(block ids, occupancy, noise values, vertex/index buffers, InstanceRaw
+records, cull masks)."""
+import sys
+
+import numpy as np
+import pytest
+
+import voxel_amd as V
+
+pytestmark = pytest.mark.gpu
+
+C1 = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=42, octaves=1, frequency=1 / 32, height_scale=16.0,
+               height_max=15)
+C2 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
+               gain=0.5, y_scale=1 / 128)
+C5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64, y_scale=1 / 128)
+REF_NOISE = V.Terrain(kind=V.TERRAIN_DENSITY, seed=1, octaves=3, frequency=0.09, lacunarity=0.5,
+                      gain=2.5, y_scale=0.0)  # the reference's own fbm parameters, threshold 0
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def oterrain(O, t: V.Terrain):
+    return O.terrain(kind=t.kind, seed=t.seed, octaves=t.octaves, frequency=t.frequency,
+                     lacunarity=t.lacunarity, gain=t.gain, height_scale=t.height_scale,
+                     height_max=t.height_max, y_scale=t.y_scale)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def assert_mesh_equal(O, mesh, ids, edge, coord, adj_ids):
+    origin = tuple(int(c) * edge for c in coord)
+    faces, inst, vert, idx = O.mesh_buffers(ids, edge, origin, adj_ids)
+    assert mesh.faces == len(faces)
+    if mesh.formats & V.FMT_INSTANCES:
+        got = mesh.instances()
+        assert (bits(got["m"]) == bits(inst["m"])).all()
+        assert (got["tex"] == inst["tex"]).all()
+    if mesh.formats & V.FMT_VERTICES:
+        gv = mesh.vertices()
+        assert (bits(gv["pos"]) == bits(vert["pos"])).all()
+        assert (bits(gv["uv"]) == bits(vert["uv"])).all()
+        assert (mesh.indices() == idx).all()
+
+
+# ------------------------------------------------------------------ noise
+@pytest.mark.parametrize("terrain,coord,edge", [
+    (C2, (5, 0, 9), 32), (C2, (-3, -1, 7), 32), (C2, (0, 0, 0), 16), (C2, (2, 1, -2), 64),
+    (C5, (100, 0, -120), 32), (REF_NOISE, (3, 0, -4), 32), (C1, (3, 0, 1), 16),
+    (V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=11, octaves=4, frequency=1 / 32,
+               height_scale=64.0, height_max=63), (-2, 0, 6), 64),
+])
+def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
+    say("noise", terrain.kind, terrain.seed, coord, edge)
+    got = ctx.noise(terrain, coord, edge)
+    ref = oracle_mod.noise_block(oterrain(oracle_mod, terrain), *coord, edge)
+    assert (bits(got) == bits(ref)).all()
+
+
+# ------------------------------------------------------------------ fill
+@pytest.mark.parametrize("terrain,coord,edge", [
+    (C1, (0, 0, 0), 16), (C1, (3, 0, 2), 16), (C1, (1, 1, 1), 16),
+    (C2, (5, 0, 9), 32), (C2, (63, 0, 63), 32), (C2, (-7, -1, 3), 32), (C2, (4, 0, 4), 16),
+    (C2, (1, 0, 1), 64), (REF_NOISE, (0, 0, 0), 32),
+    (V.Terrain(kind=V.TERRAIN_AIR), (0, 0, 0), 32), (V.Terrain(kind=V.TERRAIN_SOLID), (2, 0, 1), 64),
+])
+def test_fill_ids_and_occupancy(ctx, oracle_mod, terrain, coord, edge):
+    say("fill", terrain.kind, terrain.seed, coord, edge)
+    ch = V.Chunk.new(ctx, terrain, coord, edge)
+    ref = oracle_mod.chunk_fill(oterrain(oracle_mod, terrain), *coord, edge)
+    assert (ch.ids() == ref).all()
+    assert (ch.occupancy() == oracle_mod.occupancy(ref, edge)).all()
+
+
+# ------------------------------------------------------------------ mesh
+def neighbour_sets(rng, edge):
+    yield (None, None, None, None)
+    yield tuple((rng.random(edge ** 3) < 0.5).astype(np.uint8) for _ in range(4))
+    yield (np.ones(edge ** 3, np.uint8), None, np.zeros(edge ** 3, np.uint8), None)
+
+
+@pytest.mark.parametrize("edge", [16, 32, 64])
+def test_mesh_from_ids_random(ctx, oracle_mod, edge):
+    rng = np.random.default_rng(edge)
+    for density in (0.3, 0.7):
+        ids = (rng.random(edge ** 3) < density).astype(np.uint8)
+        for adj in neighbour_sets(rng, edge):
+            say("mesh random", edge, density, [a is None for a in adj])
+            coord = (int(rng.integers(-50, 50)), int(rng.integers(-3, 3)), int(rng.integers(-50, 50)))
+            ch = V.Chunk.from_ids(ctx, coord, edge, ids)
+            nbs = [None if a is None else V.Chunk.from_ids(ctx, (0, 0, 0), edge, a) for a in adj]
+            mesh = ch.create_mesh(nbs)
+            assert_mesh_equal(oracle_mod, mesh, ids, edge, coord, adj)
+
+
+def test_mesh_mixed_block_types(ctx, oracle_mod):
+    """Grass and dirt in one chunk: texture layers come from each voxel."""
+    rng = np.random.default_rng(5)
+    edge = 32
+    ids = rng.integers(0, 3, size=edge ** 3).astype(np.uint8)
+    ch = V.Chunk.from_ids(ctx, (1, 0, -1), edge, ids)
+    assert_mesh_equal(oracle_mod, ch.create_mesh(), ids, edge, (1, 0, -1), (None,) * 4)
+    dirt = np.where(ids > 0, 2, 0).astype(np.uint8)  # single non-grass type
+    ch2 = V.Chunk.from_ids(ctx, (0, 0, 0), edge, dirt)
+    assert_mesh_equal(oracle_mod, ch2.create_mesh(), dirt, edge, (0, 0, 0), (None,) * 4)
+
+
+@pytest.mark.parametrize("edge", [16, 32, 64])
+def test_mesh_closed_forms(ctx, oracle_mod, edge):
+    say("closed forms", edge)
+    solid = np.ones(edge ** 3, np.uint8)
+    air = np.zeros(edge ** 3, np.uint8)
+    s = V.Chunk.from_ids(ctx, (0, 0, 0), edge, solid)
+    a = V.Chunk.from_ids(ctx, (0, 0, 0), edge, air)
+    assert s.create_mesh().faces == 6 * edge * edge
+    assert s.create_mesh([s, s, s, s]).faces == 2 * edge * edge
+    assert a.create_mesh().faces == 0
+    y, z, x = np.meshgrid(np.arange(edge), np.arange(edge), np.arange(edge), indexing="ij")
+    checker = (((x + y + z) % 2) == 0).astype(np.uint8).reshape(-1)
+    c = V.Chunk.from_ids(ctx, (2, -1, 3), edge, checker)
+    m = c.create_mesh([c, c, c, c])
+    assert m.faces == 3 * edge ** 3  # the worst case: more than one LDS tile of faces
+    assert_mesh_equal(oracle_mod, m, checker, edge, (2, -1, 3), (checker,) * 4)
+
+
+def test_single_voxels_at_every_corner(ctx, oracle_mod):
+    edge = 16
+    for x in (0, edge - 1):
+        for y in (0, edge - 1):
+            for z in (0, edge - 1):
+                ids = np.zeros(edge ** 3, np.uint8)
+                ids[x + edge * z + edge * edge * y] = 1
+                ch = V.Chunk.from_ids(ctx, (-1, 0, 1), edge, ids)
+                assert_mesh_equal(oracle_mod, ch.create_mesh(), ids, edge, (-1, 0, 1), (None,) * 4)
+
+
+def test_generated_chunk_mesh_with_generated_neighbours(ctx, oracle_mod):
+    t = oterrain(oracle_mod, C2)
+    c = (6, 0, 9)
+    say("generated mesh", c)
+    ch = V.Chunk.new(ctx, C2, c, 32)
+    coords = [(c[0] - 1, 0, c[2]), (c[0] + 1, 0, c[2]), (c[0], 0, c[2] - 1), (c[0], 0, c[2] + 1)]
+    nbs = [V.Chunk.new(ctx, C2, q, 32) for q in coords]
+    ids = oracle_mod.chunk_fill(t, *c, 32)
+    adj = [oracle_mod.chunk_fill(t, *q, 32) for q in coords]
+    assert_mesh_equal(oracle_mod, ch.create_mesh(nbs), ids, 32, c, adj)
+    # neighbour snapshot with some neighbours absent (chunk_builder.rs:134-169)
+    assert_mesh_equal(oracle_mod, ch.create_mesh([nbs[0], None, None, nbs[3]]), ids, 32, c,
+                      [adj[0], None, None, adj[3]])
+
+
+# ------------------------------------------------------------------ batches
+def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES,
+                    face_capacity=0, per_chunk=False):
+    tids = [ctx.terrain(t) for t in terrains]
+    b = V.Batch(ctx, coords, edges, tids, nb, formats=formats, face_capacity=face_capacity)
+    if per_chunk:
+        b.run_per_chunk()
+    else:
+        b.run()
+    total = b.wait()
+    ids = [O.chunk_fill(oterrain(O, t), *map(int, c), int(e)) for c, e, t in zip(coords, edges, terrains)]
+    verts = b.mesh(V.FMT_VERTICES) if formats & V.FMT_VERTICES else None
+    idxs = b.mesh(V.FMT_INDICES) if formats & V.FMT_VERTICES else None
+    insts = b.mesh(V.FMT_INSTANCES) if formats & V.FMT_INSTANCES else None
+    run = 0
+    for i in range(len(coords)):
+        assert (b.ids(i) == ids[i]).all()
+        adj = [ids[j] if j >= 0 else None for j in nb[i]]
+        e = int(edges[i])
+        faces, inst, vert, idx = O.mesh_buffers(ids[i], e, tuple(int(v) * e for v in coords[i]), adj)
+        first, cnt = b.chunk_faces(i)
+        assert first == run and cnt == len(faces)
+        if verts is not None:
+            assert (bits(verts[4 * first:4 * (first + cnt)]) == bits(vert)).all()
+            assert (idxs[6 * first:6 * (first + cnt)] == idx).all()
+        if insts is not None:
+            assert (bits(insts[first:first + cnt]) == bits(inst)).all()
+        run += cnt
+    assert total == run
+    return b
+
+
+def test_batch_config1_grid(ctx, oracle_mod):
+    """Config 1: 4x4 grid of 16^3 heightmap chunks, all in-grid neighbours."""
+    say("batch C1")
+    coords = [(x, 0, z) for x in range(4) for z in range(4)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [16] * 16, [C1] * 16, V.grid_neighbours(4, 4))
+
+
+@pytest.mark.parametrize("per_chunk", [False, True])
+def test_batch_config2_subgrid(ctx, oracle_mod, per_chunk):
+    """Config 2 terrain on an 8x8 sub-grid of 32^3 chunks."""
+    say("batch C2 8x8", per_chunk)
+    coords = [(x + 20, 0, z + 30) for x in range(8) for z in range(8)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [32] * 64, [C2] * 64, V.grid_neighbours(8, 8),
+                    per_chunk=per_chunk)
+
+
+def test_batch_ragged(ctx, oracle_mod):
+    """Config 3 shape: edges from {16,32,64}, 30% air / 10% solid / 60%
+    heightmap surface (seed 11), no neighbours, longest-first schedule."""
+    say("batch ragged")
+    rng = np.random.default_rng(11)
+    n = 48
+    edges = rng.choice([16, 32, 64], size=n)
+    cls = rng.choice(3, size=n, p=[0.3, 0.1, 0.6])
+    terrains = []
+    for e, k in zip(edges, cls):
+        if k == 0:
+            terrains.append(V.Terrain(kind=V.TERRAIN_AIR))
+        elif k == 1:
+            terrains.append(V.Terrain(kind=V.TERRAIN_SOLID))
+        else:
+            terrains.append(V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=11, octaves=4,
+                                      frequency=1 / 32, height_scale=float(e), height_max=int(e) - 1))
+    coords = [(i % 8, 0, i // 8) for i in range(n)]
+    batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1))
+
+
+def test_batch_overflow_grows_arena(ctx, oracle_mod):
+    say("batch overflow")
+    coords = [(x, 0, z) for x in range(3) for z in range(3)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [32] * 9, [C2] * 9, V.grid_neighbours(3, 3),
+                    face_capacity=100)
+
+
+def test_batch_replay_is_stable(ctx, oracle_mod):
+    """hipGraph replays give identical results, with eager work between."""
+    coords = [(x, 0, z) for x in range(4) for z in range(4)]
+    b = V.Batch(ctx, coords, 32, ctx.terrain(C2), V.grid_neighbours(4, 4))
+    b.run()
+    t0 = b.wait()
+    v0 = b.mesh(V.FMT_VERTICES).copy()
+    for _ in range(3):
+        ctx.noise(C2, (0, 0, 0), 16)  # eager work between replays
+        b.run()
+        assert b.wait() == t0
+        assert (bits(b.mesh(V.FMT_VERTICES)) == bits(v0)).all()
+
+
+def test_same_alone_and_in_the_mix(ctx, oracle_mod):
+    """A request's mesh is byte-identical whether it runs alone (per-chunk
+    API) or inside a batch of other requests, given the same neighbours."""
+    coords = [(x, 0, z) for x in range(3) for z in range(3)]
+    b = V.Batch(ctx, coords, 32, ctx.terrain(C2), V.grid_neighbours(3, 3),
+                formats=V.FMT_VERTICES | V.FMT_INSTANCES)
+    b.run()
+    b.wait()
+    verts, insts = b.mesh(V.FMT_VERTICES), b.mesh(V.FMT_INSTANCES)
+    chunks = [V.Chunk.new(ctx, C2, c, 32) for c in coords]
+    nb = V.grid_neighbours(3, 3)
+    for i in range(len(coords)):
+        m = chunks[i].create_mesh([chunks[j] if j >= 0 else None for j in nb[i]])
+        first, cnt = b.chunk_faces(i)
+        assert m.faces == cnt
+        assert (bits(m.vertices()) == bits(verts[4 * first:4 * (first + cnt)])).all()
+        assert (bits(m.instances()) == bits(insts[first:first + cnt])).all()
+
+
+# ------------------------------------------------------------------ frustum
+def test_frustum_mask_matches_oracle(ctx, oracle_mod):
+    import test_oracle as T
+    rng = np.random.default_rng(12)
+    for _ in range(5):
+        eye = rng.uniform(-500, 500, 3)
+        m = T.colmajor(T.perspective(np.radians(70), 1.6, 0.1, 1024) @
+                       T.look_at(eye, eye + rng.normal(size=3)))
+        f = V.Frustum(m)
+        coords = rng.integers(-64, 64, size=(20000, 3)).astype(np.int32)
+        got = f.mask(ctx, coords, 32)
+        ref = oracle_mod.frustum_contains(f.planes, coords, 32)
+        assert (got == ref).all()
+        assert got.sum() > 0

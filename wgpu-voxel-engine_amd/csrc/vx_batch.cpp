@@ -1,0 +1,468 @@
+// vx_batch.cpp -- batched chunk generation: n requests of Chunk::new +
+// Chunk::create_mesh (src/chunk.rs:21-57) run as four kernels over the whole
+// batch (fill -> count -> scan -> emit), captured once into a hipGraph and
+// replayed.  This replaces the reference's one-job-per-chunk thread pool
+// (src/chunk_builder.rs:80-84) and its per-mesh wgpu upload
+// (src/chunk.rs:254-266): ids, occupancy and meshes stay in HBM.
+//
+// Layout in HBM
+//   store   per chunk i at chunk_off[i]: ids (N^3 B) | occupancy rows
+//           (N^2 * W u32) | 4 boundary planes (4 * N * W u32) | meta (2 u32),
+//           each 256-byte aligned.
+//   ctl     counts[n] u32 | bases[n] u64 | total u64 | overflow u32
+//   arena   instances (68 B/face) | vertices (80 B/face) | indices (24 B/face)
+//           for `capacity` faces; chunk i's faces are [bases[i], +counts[i]).
+#include "vx_host.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+using namespace vx;
+
+#define VX_HIP(ctx, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (ctx)->set_error(e_, #call);                                                    \
+            return VX_ERR_DEVICE;                                                           \
+        }                                                                                   \
+    } while (0)
+
+namespace {
+
+size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Layout {
+    size_t occ, planes, meta, total;
+};
+
+Layout layout_of(int N) {
+    const size_t W = (size_t)words_per_row(N);
+    Layout L;
+    L.occ = up256((size_t)N * N * N);
+    L.planes = L.occ + up256((size_t)N * N * W * 4);
+    L.meta = L.planes + up256(4 * (size_t)N * W * 4);
+    L.total = L.meta + 256;
+    return L;
+}
+
+// Work estimate used to schedule large chunks first in ragged batches.
+double cost_of(const vx_ctx *ctx, const vx_batch_req &r) {
+    const double n3 = (double)r.edge * r.edge * r.edge;
+    const TerrainDev &T = ctx->terrains[r.terrain_id];
+    if (T.kind == kDensity) return n3 * T.octaves;
+    if (T.kind == kHeightmap) return (double)r.edge * r.edge * T.octaves * 8.0 + n3 / 8.0;
+    return n3 / 16.0;
+}
+
+size_t arena_bytes(uint64_t cap, uint32_t formats, size_t *off_v, size_t *off_i) {
+    size_t b = 0;
+    *off_v = *off_i = 0;
+    if (formats & kFmtInstances) b = up256(cap * 68);
+    if (formats & kFmtVertices) {
+        *off_v = b;
+        b += up256(cap * 80);
+        *off_i = b;
+        b += up256(cap * 24);
+    }
+    return b ? b : 256;
+}
+
+int alloc_arena(vx_batch *b, uint64_t cap) {
+    vx_ctx *ctx = b->ctx;
+    if (b->arena) ctx->dev_free(b->arena);
+    b->arena = nullptr;
+    size_t off_v, off_i;
+    const size_t bytes = arena_bytes(cap, b->formats, &off_v, &off_i);
+    int st = ctx->dev_alloc(bytes, &b->arena);
+    if (st) {
+        b->capacity = 0;
+        b->out = MeshOut{};
+        return st;
+    }
+    char *a = (char *)b->arena;
+    b->capacity = cap;
+    b->out = MeshOut{};
+    b->out.capacity = cap;
+    b->out.formats = b->formats;
+    if (b->formats & kFmtInstances) b->out.instances = (uint32_t *)a;
+    if (b->formats & kFmtVertices) {
+        b->out.vertices = (float *)(a + off_v);
+        b->out.indices = (uint32_t *)(a + off_i);
+    }
+    return VX_OK;
+}
+
+hipError_t enqueue_fill(vx_batch *b) {
+    vx_ctx *ctx = b->ctx;
+    return launch_fill(b->fill_dev, b->order_dev, b->n, b->max_edge, ctx->terrains_dev,
+                       ctx->lut_dev, ctx->stream);
+}
+hipError_t enqueue_count(vx_batch *b) {
+    return launch_count(b->mesh_dev, b->n, b->max_edge, b->counts, b->ctx->stream);
+}
+hipError_t enqueue_scan(vx_batch *b) {
+    return launch_scan(b->counts, b->n, b->bases, b->total, b->overflow, b->ctx->stream);
+}
+hipError_t enqueue_emit(vx_batch *b) {
+    return launch_emit(b->mesh_dev, b->order_dev, b->n, b->max_edge, b->bases, b->counts, b->out,
+                       b->ctx->tables_dev, b->overflow, b->ctx->stream);
+}
+
+hipError_t enqueue_all(vx_batch *b) {
+    hipError_t e = enqueue_fill(b);
+    if (e == hipSuccess) e = enqueue_count(b);
+    if (e == hipSuccess) e = enqueue_scan(b);
+    if (e == hipSuccess) e = enqueue_emit(b);
+    return e;
+}
+
+void drop_graph(vx_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+int capture_graph(vx_batch *b) {
+    vx_ctx *ctx = b->ctx;
+    drop_graph(b);
+    VX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_all(b);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        ctx->set_error(e != hipSuccess ? e : e2, "batch graph capture");
+        return VX_ERR_DEVICE;
+    }
+    b->graph = g;
+    VX_HIP(ctx, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    return VX_OK;
+}
+
+void free_batch(vx_batch *b) {
+    vx_ctx *ctx = b->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    drop_graph(b);
+    ctx->dev_free(b->store);
+    ctx->dev_free(b->fill_dev);
+    ctx->dev_free(b->mesh_dev);
+    ctx->dev_free(b->order_dev);
+    ctx->dev_free(b->ctl);
+    ctx->dev_free(b->arena);
+    delete b;
+}
+
+int fetch_results(vx_batch *b) {
+    vx_ctx *ctx = b->ctx;
+    if (b->results) return VX_OK;
+    b->h_counts.resize(b->n);
+    b->h_bases.resize(b->n);
+    VX_HIP(ctx, hipMemcpyAsync(b->h_counts.data(), b->counts, (size_t)b->n * 4,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipMemcpyAsync(b->h_bases.data(), b->bases, (size_t)b->n * 8,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    b->results = true;
+    return VX_OK;
+}
+
+}  // namespace
+
+void vx_batch_unref(vx_batch *b) {
+    if (b->refs.fetch_sub(1) == 1) free_batch(b);
+}
+
+extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n, uint32_t formats,
+                               uint64_t face_capacity, vx_batch **out) {
+    if (!ctx || !out || n <= 0 || !reqs) return VX_ERR_INVALID;
+    if (formats == 0 || (formats & ~(kFmtInstances | kFmtVertices))) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    for (int i = 0; i < n; i++) {
+        const vx_batch_req &r = reqs[i];
+        if (r.edge != 16 && r.edge != 32 && r.edge != 64) return VX_ERR_INVALID;
+        if (r.terrain_id < 0 || r.terrain_id >= (int)ctx->terrains.size()) return VX_ERR_INVALID;
+        for (int k = 0; k < 4; k++) {
+            if (r.nb[k] < -1 || r.nb[k] >= n || r.nb[k] == i) return VX_ERR_INVALID;
+            if (r.nb[k] >= 0 && reqs[r.nb[k]].edge != r.edge) return VX_ERR_INVALID;
+        }
+    }
+    vx_batch *b = new (std::nothrow) vx_batch();
+    if (!b) return VX_ERR_NOMEM;
+    b->ctx = ctx;
+    b->n = n;
+    b->formats = formats;
+    b->reqs.assign(reqs, reqs + n);
+    b->chunk_off.resize(n);
+    size_t off = 0;
+    int emin = 64;
+    for (int i = 0; i < n; i++) {
+        b->chunk_off[i] = off;
+        off += layout_of(reqs[i].edge).total;
+        b->max_edge = std::max(b->max_edge, reqs[i].edge);
+        emin = std::min(emin, reqs[i].edge);
+    }
+    // schedule: identical requests keep launch order; otherwise longest first
+    std::vector<double> cost(n);
+    for (int i = 0; i < n; i++) cost[i] = cost_of(ctx, reqs[i]);
+    b->uniform = emin == b->max_edge &&
+                 std::all_of(cost.begin(), cost.end(), [&](double c) { return c == cost[0]; });
+    int st = VX_OK;
+    auto fail = [&](int s) {
+        free_batch(b);
+        return s;
+    };
+    if ((st = ctx->dev_alloc(off, &b->store))) return fail(st);
+    if ((st = ctx->dev_alloc((size_t)n * sizeof(FillWork), (void **)&b->fill_dev))) return fail(st);
+    if ((st = ctx->dev_alloc((size_t)n * sizeof(MeshWork), (void **)&b->mesh_dev))) return fail(st);
+    const size_t ctl_bytes = up256((size_t)n * 4) + up256((size_t)n * 8) + 256;
+    if ((st = ctx->dev_alloc(ctl_bytes, &b->ctl))) return fail(st);
+    b->counts = (uint32_t *)b->ctl;
+    b->bases = (uint64_t *)((char *)b->ctl + up256((size_t)n * 4));
+    b->total = (uint64_t *)((char *)b->bases + up256((size_t)n * 8));
+    b->overflow = (uint32_t *)(b->total + 1);
+
+    std::vector<FillWork> fw(n);
+    std::vector<MeshWork> mw(n);
+    char *store = (char *)b->store;
+    for (int i = 0; i < n; i++) {
+        const vx_batch_req &r = reqs[i];
+        const Layout L = layout_of(r.edge);
+        char *base = store + b->chunk_off[i];
+        FillWork &f = fw[i];
+        f = FillWork{};
+        f.cx = r.coord.x;
+        f.cy = r.coord.y;
+        f.cz = r.coord.z;
+        f.edge = r.edge;
+        f.terrain = r.terrain_id;
+        f.ids = (uint8_t *)base;
+        f.occ = (uint32_t *)(base + L.occ);
+        f.planes = (uint32_t *)(base + L.planes);
+        f.meta = (uint32_t *)(base + L.meta);
+    }
+    for (int i = 0; i < n; i++) {
+        const vx_batch_req &r = reqs[i];
+        MeshWork &m = mw[i];
+        m = MeshWork{};
+        m.ox = r.coord.x * r.edge;
+        m.oy = r.coord.y * r.edge;
+        m.oz = r.coord.z * r.edge;
+        m.edge = r.edge;
+        m.occ = fw[i].occ;
+        m.ids = fw[i].ids;
+        m.solid_type = kBlockGrass;  // generated chunks hold only grass
+        static const int which[4] = {1, 0, 3, 2};  // facing plane of the neighbour
+        for (int k = 0; k < 4; k++) {
+            const int j = r.nb[k];
+            m.nb[k] = j >= 0 ? fw[j].planes + (size_t)which[k] * r.edge * words_per_row(r.edge)
+                             : nullptr;
+        }
+    }
+    VX_HIP(ctx, hipMemcpyAsync(b->fill_dev, fw.data(), fw.size() * sizeof(FillWork),
+                               hipMemcpyHostToDevice, ctx->stream));
+    VX_HIP(ctx, hipMemcpyAsync(b->mesh_dev, mw.data(), mw.size() * sizeof(MeshWork),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!b->uniform) {
+        std::vector<int32_t> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int c) { return cost[a] > cost[c]; });
+        if ((st = ctx->dev_alloc((size_t)n * 4, (void **)&b->order_dev))) return fail(st);
+        VX_HIP(ctx, hipMemcpyAsync(b->order_dev, order.data(), (size_t)n * 4,
+                                   hipMemcpyHostToDevice, ctx->stream));
+    }
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // host vectors are pageable
+    uint64_t cap = face_capacity;
+    if (cap == 0) {
+        for (int i = 0; i < n; i++) cap += 4ull * reqs[i].edge * reqs[i].edge;
+    }
+    if ((st = alloc_arena(b, cap))) return fail(st);
+    *out = b;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_destroy(vx_batch *b) {
+    if (!b) return VX_ERR_INVALID;
+    vx_ctx *ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    vx_batch_unref(b);
+    return VX_OK;
+}
+
+extern "C" int vx_batch_run(vx_ctx *ctx, vx_batch *b) {
+    if (!ctx || !b || b->ctx != ctx) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!b->graph_exec) {
+        int st = capture_graph(b);
+        if (st) return st;
+    }
+    VX_HIP(ctx, hipGraphLaunch(b->graph_exec, ctx->stream));
+    b->ran = true;
+    b->results = false;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *b) {
+    if (!ctx || !b || b->ctx != ctx) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    const int n = b->n;
+    for (int i = 0; i < n; i++)
+        VX_HIP(ctx, launch_fill(b->fill_dev + i, nullptr, 1, b->reqs[i].edge, ctx->terrains_dev,
+                                ctx->lut_dev, ctx->stream));
+    for (int i = 0; i < n; i++)
+        VX_HIP(ctx, launch_count(b->mesh_dev + i, 1, b->reqs[i].edge, b->counts + i, ctx->stream));
+    VX_HIP(ctx, enqueue_scan(b));
+    for (int i = 0; i < n; i++)
+        VX_HIP(ctx, launch_emit(b->mesh_dev + i, nullptr, 1, b->reqs[i].edge, b->bases + i,
+                                b->counts + i, b->out, ctx->tables_dev, b->overflow, ctx->stream));
+    b->ran = true;
+    b->results = false;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_wait(vx_ctx *ctx, vx_batch *b, uint64_t *total_faces) {
+    if (!ctx || !b || b->ctx != ctx) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!b->ran) return VX_ERR_NOT_READY;
+    VX_HIP(ctx, hipMemcpyAsync(ctx->pinned, b->total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t total;
+    uint32_t overflow;
+    memcpy(&total, ctx->pinned, 8);
+    memcpy(&overflow, ctx->pinned + 8, 4);
+    if (overflow || total > b->capacity) {
+        // grow the arena to the exact total and re-run only the emit stage;
+        // counts and bases from this run stay valid
+        int st = alloc_arena(b, total + total / 16 + 1024);
+        if (st) return st;
+        drop_graph(b);  // arena pointers are baked into the captured graph
+        VX_HIP(ctx, hipMemsetAsync(b->overflow, 0, 4, ctx->stream));
+        VX_HIP(ctx, enqueue_emit(b));
+        VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy(&overflow, ctx->pinned + 8, 4);
+        if (overflow) return VX_ERR_CAPACITY;
+    }
+    b->h_total = total;
+    b->results = false;
+    if (total_faces) *total_faces = total;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_chunk_faces(vx_ctx *ctx, vx_batch *b, int32_t i, uint64_t *first,
+                                    uint64_t *count) {
+    if (!ctx || !b || i < 0 || i >= b->n) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!b->ran) return VX_ERR_NOT_READY;
+    int st = fetch_results(b);
+    if (st) return st;
+    if (first) *first = b->h_bases[i];
+    if (count) *count = b->h_counts[i];
+    return VX_OK;
+}
+
+extern "C" int vx_batch_read_ids(vx_ctx *ctx, vx_batch *b, int32_t i, uint8_t *dst, uint64_t cap) {
+    if (!ctx || !b || !dst || i < 0 || i >= b->n) return VX_ERR_INVALID;
+    const uint64_t nv = (uint64_t)b->reqs[i].edge * b->reqs[i].edge * b->reqs[i].edge;
+    if (cap < nv) return VX_ERR_CAPACITY;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    VX_HIP(ctx, hipMemcpyAsync(dst, (char *)b->store + b->chunk_off[i], nv, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VX_OK;
+}
+
+extern "C" int vx_batch_read_mesh(vx_ctx *ctx, vx_batch *b, uint32_t format, void *dst,
+                                  uint64_t cap) {
+    if (!ctx || !b || !dst) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!b->ran) return VX_ERR_NOT_READY;
+    const uint64_t F = b->h_total;
+    const void *src;
+    uint64_t bytes;
+    if (format == kFmtInstances && (b->formats & kFmtInstances)) {
+        src = b->out.instances;
+        bytes = F * 68;
+    } else if (format == kFmtVertices && (b->formats & kFmtVertices)) {
+        src = b->out.vertices;
+        bytes = F * 80;
+    } else if (format == 4u && (b->formats & kFmtVertices)) {  // indices
+        src = b->out.indices;
+        bytes = F * 24;
+    } else {
+        return VX_ERR_INVALID;
+    }
+    if (cap < bytes) return VX_ERR_CAPACITY;
+    if (!bytes) return VX_OK;
+    VX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VX_OK;
+}
+
+extern "C" int vx_batch_device_buffers(vx_batch *b, void **inst, void **vert, void **idx) {
+    if (!b) return VX_ERR_INVALID;
+    if (inst) *inst = b->out.instances;
+    if (vert) *vert = b->out.vertices;
+    if (idx) *idx = b->out.indices;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_profile(vx_ctx *ctx, vx_batch *b, int32_t reps, float ms[5]) {
+    if (!ctx || !b || !ms || reps <= 0) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; i++) VX_HIP(ctx, hipEventCreate(&ev[i]));
+    double acc[5] = {0, 0, 0, 0, 0};
+    int st = VX_OK;
+    for (int r = 0; r < reps && !st; r++) {
+        hipError_t e = hipEventRecord(ev[0], ctx->stream);
+        if (e == hipSuccess) e = enqueue_fill(b);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
+        if (e == hipSuccess) e = enqueue_count(b);
+        if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
+        if (e == hipSuccess) e = enqueue_scan(b);
+        if (e == hipSuccess) e = hipEventRecord(ev[3], ctx->stream);
+        if (e == hipSuccess) e = enqueue_emit(b);
+        if (e == hipSuccess) e = hipEventRecord(ev[4], ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
+        if (e != hipSuccess) {
+            ctx->set_error(e, "vx_batch_profile");
+            st = VX_ERR_DEVICE;
+            break;
+        }
+        for (int k = 0; k < 4; k++) {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            acc[k] += t;
+        }
+    }
+    if (!st) {
+        if (!b->graph_exec) st = capture_graph(b);
+        if (!st) {
+            hipError_t e = hipEventRecord(ev[0], ctx->stream);
+            for (int r = 0; r < reps && e == hipSuccess; r++)
+                e = hipGraphLaunch(b->graph_exec, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[5], ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
+            if (e != hipSuccess) {
+                ctx->set_error(e, "vx_batch_profile graph");
+                st = VX_ERR_DEVICE;
+            } else {
+                float t = 0;
+                (void)hipEventElapsedTime(&t, ev[0], ev[5]);
+                acc[4] = t;
+            }
+        }
+    }
+    for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+    if (st) return st;
+    for (int k = 0; k < 5; k++) ms[k] = (float)(acc[k] / reps);
+    b->ran = true;
+    b->results = false;
+    return VX_OK;
+}

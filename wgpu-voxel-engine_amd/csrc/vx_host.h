@@ -1,0 +1,122 @@
+// vx_host.h -- host-side object layouts behind the opaque handles of
+// include/voxel.h (context, chunk, mesh, batch) and the caching device heap.
+#pragma once
+
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/voxel.h"
+#include "vx_internal.h"
+
+namespace vx {
+
+constexpr int kMaxTerrains = 64;
+
+// Caching device allocator: blocks are rounded to size classes (4 per power
+// of two, >= 256 B) and recycled through per-class free lists, so the chunk
+// and mesh churn of a streaming world never reaches hipMalloc/hipFree after
+// warm-up.  live() counts bytes handed out (what the HBM budget limits).
+class DeviceHeap {
+  public:
+    static size_t size_class(size_t n);
+    hipError_t alloc(size_t n, void **out);
+    void release(void *p);
+    void trim();
+    uint64_t live() const { return live_; }
+    uint64_t cached() const { return cached_; }
+    ~DeviceHeap();
+
+  private:
+    std::map<size_t, std::vector<void *>> free_;
+    std::unordered_map<void *, size_t> sizes_;
+    uint64_t live_ = 0, cached_ = 0;
+};
+
+}  // namespace vx
+
+struct vx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    uint64_t budget = 0;
+    std::string last_error;
+    vx::DeviceHeap heap;
+    std::vector<vx::TerrainDev> terrains;
+    vx::TerrainDev *terrains_dev = nullptr;
+    vx::GradLut *lut_dev = nullptr;
+    vx::MeshTables *tables_dev = nullptr;
+    void *small_dev = nullptr;  // [0] face count, [1] overflow flag of the per-chunk path
+    void *scratch = nullptr;    // staging for host-array entry points
+    size_t scratch_bytes = 0;
+    unsigned char *pinned = nullptr;  // 4 KiB pinned host staging for readbacks
+
+    void set_error(hipError_t e, const char *what);
+    int dev_alloc(size_t n, void **out);
+    void dev_free(void *p);
+    int ensure_scratch(size_t bytes);
+};
+
+struct vx_batch;
+
+struct vx_chunk {
+    std::atomic<int> refs{1};
+    vx_ctx *ctx = nullptr;
+    vx_coord coord{0, 0, 0};
+    int32_t edge = 0;
+    void *block = nullptr;       // owned device block (null when a view into a batch)
+    vx_batch *owner = nullptr;   // batch the storage belongs to (retained)
+    uint8_t *ids = nullptr;
+    uint32_t *occ = nullptr;
+    uint32_t *planes = nullptr;
+    uint32_t *meta = nullptr;
+    uint32_t solid_type = vx::kBlockGrass;
+};
+
+struct vx_mesh {
+    std::atomic<int> refs{1};
+    vx_ctx *ctx = nullptr;
+    uint64_t faces = 0;
+    uint32_t formats = 0;
+    void *block = nullptr;
+    vx_batch *owner = nullptr;
+    float *vertices = nullptr;
+    uint32_t *indices = nullptr;
+    uint32_t *instances = nullptr;
+};
+
+struct vx_batch {
+    std::atomic<int> refs{1};
+    vx_ctx *ctx = nullptr;
+    int32_t n = 0;
+    int32_t max_edge = 0;
+    uint32_t formats = 0;
+    bool uniform = true;
+    std::vector<vx_batch_req> reqs;
+    std::vector<size_t> chunk_off;  // byte offset of chunk i in `store`
+    void *store = nullptr;          // ids | occ | planes | meta per chunk
+    vx::FillWork *fill_dev = nullptr;
+    vx::MeshWork *mesh_dev = nullptr;
+    int32_t *order_dev = nullptr;   // longest-first schedule (null when uniform)
+    void *ctl = nullptr;            // counts[n] | bases[n] | total | overflow
+    uint32_t *counts = nullptr;
+    uint64_t *bases = nullptr;
+    uint64_t *total = nullptr;
+    uint32_t *overflow = nullptr;
+    uint64_t capacity = 0;          // faces the mesh arena holds
+    void *arena = nullptr;
+    vx::MeshOut out{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool ran = false;
+    bool results = false;           // host copies below are current
+    uint64_t h_total = 0;
+    std::vector<uint32_t> h_counts;
+    std::vector<uint64_t> h_bases;
+};
+
+void vx_batch_unref(vx_batch *b);
+void vx_chunk_destroy_locked(vx_chunk *c);

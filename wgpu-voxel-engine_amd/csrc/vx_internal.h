@@ -1,0 +1,121 @@
+// vx_internal.h -- structures shared by the host runtime (vx_api.cpp) and the
+// gfx950 kernels (vx_kernels.hip).  Plain-old-data only: every struct here is
+// copied byte-for-byte into device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vx {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kMaxOctaves = 8;     // fBm octaves supported by the noise tables
+constexpr int kDescTile = 4096;    // face descriptors staged in LDS per emit tile
+
+// Terrain kinds (mirror of vx_terrain.kind in include/voxel.h).
+enum : int32_t { kAir = 0, kSolid = 1, kHeightmap = 2, kDensity = 3 };
+
+// Block ids stored in the u8 voxel array: 0 air, 1 grass, 2 dirt.  The
+// reference's CubeType is {GRASS=0, DIRT=1, AIR=2} (src/cube.rs:16-21).
+enum : uint32_t { kBlockAir = 0, kBlockGrass = 1, kBlockDirt = 2, kBlockMixed = 0xFFu };
+
+// Output formats (mirror of VX_FMT_* in include/voxel.h).
+enum : uint32_t { kFmtInstances = 1u, kFmtVertices = 2u };
+
+// Per-terrain constants, resident in device memory.  The permutation is the
+// splitmix64/Fisher-Yates shuffle of 0..255 for the seed, doubled to 512.
+struct TerrainDev {
+    int32_t kind;
+    int32_t octaves;
+    float frequency;
+    float lacunarity;
+    float gain;
+    float height_scale;
+    int32_t height_max;
+    float y_scale;
+    int32_t perm[512];
+};
+
+// Per-hash gradient coefficients for the column-hoisted noise evaluation,
+// indexed [corner k][hash & 15].  grad_k = fma(yr, gy, ax*xr + az*zr); the
+// +-1 entries select and sign an operand, the zero entries are the
+// signed zero that makes the product vanish exactly (-0 when the operand is
+// >= 0, +0 when it is < 0), so the sum equals Perlin's (+-u) + (+-v) bit for
+// bit.
+struct GradLut {
+    float4 c[8][16];  // (ax, az, gy, unused)
+};
+
+// Meshing constants computed on the host from the rotation of each face
+// direction (nalgebra from_euler_angles restated) and the translation sign:
+//   P[d][s][corner][r] = ((M(r,0)*cx + M(r,1)*cy) + M(r,2)*cz)
+//   Mb[d][s][r][c]     = M(r,c) of T*R for a translation component of sign s
+// where s = 1 iff that translation component is negative.
+struct MeshTables {
+    float P[6][2][4][3];
+    float Mb[6][2][3][3];
+    uint32_t tex[3][6];  // [block id][dir] -> texture layer (row 0 unused)
+};
+
+// One chunk to fill: terrain -> ids, occupancy rows, boundary planes, meta.
+struct FillWork {
+    int32_t cx, cy, cz, edge;
+    int32_t terrain;       // index into the terrain table
+    int32_t pad0;
+    uint8_t *ids;          // edge^3
+    uint32_t *occ;         // edge^2 rows * W words (W = 1 for edge <= 32, 2 for 64)
+    uint32_t *planes;      // 4 planes * edge * W words: x=0, x=N-1, z=0, z=N-1
+    uint32_t *meta;        // [0] solid block id (or kBlockMixed), [1] solid voxel count
+    float *noise_out;      // optional raw noise (density: edge^3, heightmap: edge^2)
+};
+
+// One chunk to mesh.  nb[k] points at the neighbour's plane facing this
+// chunk (k = -x, +x, -z, +z -> its x=N-1, x=0, z=N-1, z=0 plane) or is null
+// when the neighbour is absent (chunk.rs:89-107: absent -> emit).
+struct MeshWork {
+    int32_t ox, oy, oz, edge;  // world origin of the chunk
+    const uint32_t *occ;
+    const uint8_t *ids;
+    const uint32_t *nb[4];
+    uint32_t solid_type;       // block id of every solid voxel, or kBlockMixed
+    uint32_t pad0;
+};
+
+// Batch-wide mesh outputs.
+struct MeshOut {
+    float *vertices;      // 20 floats per face (4 x {pos xyz, uv})
+    uint32_t *indices;    // 6 per face, chunk-local
+    uint32_t *instances;  // 17 dwords per face (InstanceRaw)
+    uint64_t capacity;    // faces
+    uint32_t formats;
+    uint32_t pad0;
+};
+
+// Six frustum planes [near far left right top bottom] x (a b c d), passed
+// to the cull kernel by value.
+struct Planes {
+    float p[24];
+};
+
+inline int words_per_row(int edge) { return (edge + 31) / 32; }
+
+// ---- kernel launchers (vx_kernels.hip) --------------------------------
+hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
+                       const TerrainDev *terrains, const GradLut *lut, hipStream_t s);
+hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint32_t *counts,
+                        hipStream_t s);
+hipError_t launch_scan(const uint32_t *counts, int32_t n, uint64_t *bases, uint64_t *total,
+                       uint32_t *overflow, hipStream_t s);
+hipError_t launch_fill_one(const FillWork &w, const TerrainDev *terrains, const GradLut *lut,
+                           hipStream_t s);
+hipError_t launch_pack_one(const FillWork &w, hipStream_t s);
+hipError_t launch_count_one(const MeshWork &w, uint32_t *count, hipStream_t s);
+hipError_t launch_emit_one(const MeshWork &w, uint64_t base, uint32_t faces, MeshOut out,
+                           const MeshTables *tables, uint32_t *overflow, hipStream_t s);
+hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t max_edge,
+                       const uint64_t *bases, const uint32_t *counts, MeshOut out,
+                       const MeshTables *tables, uint32_t *overflow, hipStream_t s);
+hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n, int32_t edge,
+                          uint8_t *mask, hipStream_t s);
+
+}  // namespace vx
