@@ -220,6 +220,103 @@ int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
  * stage (no batching, no graph), chunk order preserved. */
 int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *batch);
 
+/* ---- Serving: continuous batching of streamed chunk requests ------------ */
+/* The reference's ChunkGenerator submits one chunk job per tick to a thread
+ * pool and collects one result per tick (src/chunk_builder.rs:69-87,
+ * 310-332).  A vx_server instead admits whatever requests are pending into
+ * the next device batch as soon as an in-flight slot and HBM budget are
+ * free, runs up to max_inflight batches concurrently on separate streams,
+ * and keeps chunks + meshes resident in an LRU cache bounded by the
+ * context's HBM budget (evicting least-recently-used entries on admission,
+ * like filter_unseen_chunks drops chunks outside the box,
+ * src/chunk_builder.rs:334-352). */
+typedef struct vx_server vx_server;
+
+/* Neighbour policy of a request: mesh standalone (absent neighbours, every
+ * boundary face emitted), or snapshot the resident neighbours of the same
+ * edge when the request is admitted (adjacent_chunks,
+ * src/chunk_builder.rs:134-169). */
+#define VX_NB_NONE 0u
+#define VX_NB_RESIDENT 1u
+
+typedef struct vx_request {
+    vx_coord coord;
+    int32_t edge;
+    int32_t terrain_id;
+    uint32_t neighbours; /* VX_NB_* */
+    uint32_t reserved;
+    uint64_t arrival_ns; /* CLOCK_MONOTONIC ns of arrival; 0 = at submission */
+} vx_request;
+
+typedef struct vx_completion {
+    uint64_t ticket;
+    vx_coord coord;
+    int32_t edge;
+    int32_t status;      /* VX_OK, or the error that failed the request */
+    uint32_t cache_hit;  /* 1 when served from the resident cache */
+    uint64_t faces;
+    uint64_t arrival_ns; /* as submitted */
+    uint64_t done_ns;    /* CLOCK_MONOTONIC ns at which the result was ready */
+    int32_t nb_present[4]; /* neighbour snapshot used: -x, +x, -z, +z */
+} vx_completion;
+
+typedef struct vx_server_desc {
+    int32_t max_batch;    /* requests per device batch (0 -> 512) */
+    int32_t max_inflight; /* batches on the device at once (0 -> 4) */
+    uint32_t formats;     /* VX_FMT_* kept resident (0 -> VX_FMT_VERTICES) */
+    uint32_t flags;       /* reserved, 0 */
+} vx_server_desc;
+
+typedef struct vx_server_stats {
+    uint64_t submitted, completed, cache_hits, coalesced, generated, batches, evictions,
+        failed;
+    uint64_t resident_chunks, resident_bytes, peak_resident_bytes, budget_bytes;
+} vx_server_stats;
+
+int vx_server_create(vx_ctx *ctx, const vx_server_desc *desc, vx_server **out);
+int vx_server_destroy(vx_server *server);
+/* Queues n requests; tickets[i] identifies request i in completions. */
+int vx_server_submit(vx_server *server, const vx_request *reqs, int32_t n, uint64_t *tickets);
+/* Makes progress without blocking longer than wait_us: retires finished
+ * device work, admits pending requests.  Returns the number of completions
+ * ready to poll (>= 0) or an error status. */
+int vx_server_pump(vx_server *server, int32_t wait_us);
+int vx_server_poll(vx_server *server, vx_completion *out, int32_t cap, int32_t *n);
+/* Blocks until every submitted request has completed. */
+int vx_server_drain(vx_server *server);
+/* Resident chunk + mesh for (coord, edge), retained for the caller (either
+ * output may be NULL); VX_ERR_NOT_READY when not resident. */
+int vx_server_lookup(vx_server *server, vx_coord coord, int32_t edge, vx_chunk **chunk,
+                     vx_mesh **mesh);
+int vx_server_stats_get(vx_server *server, vx_server_stats *stats);
+
+/* Streaming simulation: a camera on a seeded random walk, Poisson request
+ * arrivals around it within the render radius, mixed chunk edges, served
+ * by a vx_server in real time; latency = completion - scheduled arrival. */
+typedef struct vx_sim_desc {
+    uint64_t seed;
+    double rate_per_s;          /* Poisson arrival rate */
+    double duration_s;          /* arrivals are generated for this long */
+    double speed_blocks_per_s;  /* camera speed */
+    int32_t radius_chunks;      /* render radius in 32-voxel chunks */
+    int32_t n_classes;          /* chunk size classes (1..3) */
+    int32_t edges[3];
+    float weights[3];
+    int32_t terrain_id[3];
+    uint32_t neighbours;        /* VX_NB_* for every request */
+    vx_server_desc server;
+} vx_sim_desc;
+
+typedef struct vx_sim_result {
+    uint64_t requests, completed, cache_hits, generated, batches, evictions, faces;
+    double wall_s, throughput_rps, generated_per_s;
+    double p50_ms, p99_ms, max_ms, mean_ms;          /* all requests */
+    double miss_p50_ms, miss_p99_ms, miss_max_ms;    /* requests that generated */
+    uint64_t peak_resident_bytes;
+} vx_sim_result;
+
+int vx_serve_simulate(vx_ctx *ctx, const vx_sim_desc *desc, vx_sim_result *result);
+
 #ifdef __cplusplus
 }
 #endif

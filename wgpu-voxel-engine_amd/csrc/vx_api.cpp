@@ -537,10 +537,17 @@ extern "C" int vx_chunk_retain(vx_chunk *c) {
     return VX_OK;
 }
 
+void vx_shared_unref_locked(vx_ctx *ctx, vx::SharedBlock *s) {
+    if (s && s->refs.fetch_sub(1) == 1) {
+        ctx->dev_free(s->ptr);
+        delete s;
+    }
+}
+
 void vx_chunk_destroy_locked(vx_chunk *c) {
     vx_ctx *ctx = c->ctx;
     if (c->block) ctx->dev_free(c->block);
-    if (c->owner) vx_batch_unref(c->owner);
+    vx_shared_unref_locked(ctx, c->shared);
     delete c;
 }
 
@@ -716,7 +723,7 @@ extern "C" int vx_mesh_release(vx_mesh *m) {
     if (m->refs.fetch_sub(1) == 1) {
         std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
         if (m->block) m->ctx->dev_free(m->block);
-        if (m->owner) vx_batch_unref(m->owner);
+        vx_shared_unref_locked(m->ctx, m->shared);
         delete m;
     }
     return VX_OK;

@@ -36,6 +36,14 @@ class DeviceHeap {
     uint64_t live_ = 0, cached_ = 0;
 };
 
+// A device block shared by several handles (the chunk views and mesh views
+// of one admitted batch).  The last release returns it to the heap; callers
+// hold the context lock.
+struct SharedBlock {
+    std::atomic<int> refs{1};
+    void *ptr = nullptr;
+};
+
 }  // namespace vx
 
 struct vx_ctx {
@@ -67,8 +75,8 @@ struct vx_chunk {
     vx_ctx *ctx = nullptr;
     vx_coord coord{0, 0, 0};
     int32_t edge = 0;
-    void *block = nullptr;       // owned device block (null when a view into a batch)
-    vx_batch *owner = nullptr;   // batch the storage belongs to (retained)
+    void *block = nullptr;              // owned device block (null for a view)
+    vx::SharedBlock *shared = nullptr;  // block this view points into (retained)
     uint8_t *ids = nullptr;
     uint32_t *occ = nullptr;
     uint32_t *planes = nullptr;
@@ -81,8 +89,8 @@ struct vx_mesh {
     vx_ctx *ctx = nullptr;
     uint64_t faces = 0;
     uint32_t formats = 0;
-    void *block = nullptr;
-    vx_batch *owner = nullptr;
+    void *block = nullptr;              // owned device block (null for a view)
+    vx::SharedBlock *shared = nullptr;  // block this view points into (retained)
     float *vertices = nullptr;
     uint32_t *indices = nullptr;
     uint32_t *instances = nullptr;
@@ -120,3 +128,4 @@ struct vx_batch {
 
 void vx_batch_unref(vx_batch *b);
 void vx_chunk_destroy_locked(vx_chunk *c);
+void vx_shared_unref_locked(vx_ctx *ctx, vx::SharedBlock *s);

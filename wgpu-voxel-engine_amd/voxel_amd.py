@@ -66,6 +66,44 @@ class FrustumC(C.Structure):
     _fields_ = [("planes", C.c_float * 24)]
 
 
+class Request(C.Structure):
+    _fields_ = [("coord", Coord), ("edge", C.c_int32), ("terrain_id", C.c_int32),
+                ("neighbours", C.c_uint32), ("reserved", C.c_uint32), ("arrival_ns", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("coord", Coord), ("edge", C.c_int32),
+                ("status", C.c_int32), ("cache_hit", C.c_uint32), ("faces", C.c_uint64),
+                ("arrival_ns", C.c_uint64), ("done_ns", C.c_uint64),
+                ("nb_present", C.c_int32 * 4)]
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("max_inflight", C.c_int32), ("formats", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "submitted", "completed", "cache_hits", "coalesced", "generated", "batches", "evictions",
+        "failed", "resident_chunks", "resident_bytes", "peak_resident_bytes", "budget_bytes")]
+
+
+class SimDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("rate_per_s", C.c_double), ("duration_s", C.c_double),
+                ("speed_blocks_per_s", C.c_double), ("radius_chunks", C.c_int32),
+                ("n_classes", C.c_int32), ("edges", C.c_int32 * 3), ("weights", C.c_float * 3),
+                ("terrain_id", C.c_int32 * 3), ("neighbours", C.c_uint32), ("server", ServerDesc)]
+
+
+class SimResult(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "requests", "completed", "cache_hits", "generated", "batches", "evictions", "faces")] + \
+        [(n, C.c_double) for n in (
+            "wall_s", "throughput_rps", "generated_per_s", "p50_ms", "p99_ms", "max_ms", "mean_ms",
+            "miss_p50_ms", "miss_p99_ms", "miss_max_ms")] + [("peak_resident_bytes", C.c_uint64)]
+
+
 class BatchReq(C.Structure):
     _fields_ = [("coord", Coord), ("edge", C.c_int32), ("terrain_id", C.c_int32),
                 ("nb", C.c_int32 * 4)]
@@ -114,6 +152,15 @@ _SIGS = {
     "vx_batch_read_mesh": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64]),
     "vx_batch_device_buffers": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
     "vx_batch_profile": (C.c_int, [P, P, C.c_int32, C.POINTER(C.c_float)]),
+    "vx_server_create": (C.c_int, [P, C.POINTER(ServerDesc), C.POINTER(P)]),
+    "vx_server_destroy": (C.c_int, [P]),
+    "vx_server_submit": (C.c_int, [P, P, C.c_int32, P]),
+    "vx_server_pump": (C.c_int, [P, C.c_int32]),
+    "vx_server_poll": (C.c_int, [P, P, C.c_int32, C.POINTER(C.c_int32)]),
+    "vx_server_drain": (C.c_int, [P]),
+    "vx_server_lookup": (C.c_int, [P, Coord, C.c_int32, C.POINTER(P), C.POINTER(P)]),
+    "vx_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "vx_serve_simulate": (C.c_int, [P, C.POINTER(SimDesc), C.POINTER(SimResult)]),
 }
 
 
@@ -435,6 +482,94 @@ class Batch:
         self.ctx.check(self.ctx.L.vx_batch_profile(self.ctx.h, self.h, reps, ms),
                        "vx_batch_profile")
         return np.array(ms[:], np.float64)
+
+
+NB_NONE, NB_RESIDENT = 0, 1
+
+
+class Server:
+    """Continuous-batching chunk server with a device-resident LRU cache
+    (the reference's ChunkGenerator + World maps, src/chunk_builder.rs:45-352)."""
+
+    def __init__(self, ctx: Context, max_batch: int = 0, max_inflight: int = 0,
+                 formats: int = FMT_VERTICES):
+        h = P()
+        ctx.check(ctx.L.vx_server_create(ctx.h, C.byref(ServerDesc(max_batch, max_inflight,
+                                                                    formats, 0)), C.byref(h)),
+                  "vx_server_create")
+        self.ctx, self.h, self.formats = ctx, h, formats
+
+    def close(self):
+        if self.h:
+            self.ctx.L.vx_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, reqs) -> list[int]:
+        """reqs: iterable of (coord, edge, terrain_id, neighbours)."""
+        reqs = list(reqs)
+        arr = (Request * len(reqs))()
+        for i, (c, e, t, nbp) in enumerate(reqs):
+            arr[i].coord = Coord(*c)
+            arr[i].edge, arr[i].terrain_id, arr[i].neighbours = e, t, nbp
+        tickets = (C.c_uint64 * len(reqs))()
+        self.ctx.check(self.ctx.L.vx_server_submit(self.h, C.cast(arr, P), len(reqs),
+                                                   C.cast(tickets, P)), "vx_server_submit")
+        return list(tickets)
+
+    def pump(self, wait_us: int = 0) -> int:
+        st = self.ctx.L.vx_server_pump(self.h, wait_us)
+        if st < 0:
+            self.ctx.check(st, "vx_server_pump")
+        return st
+
+    def drain(self):
+        self.ctx.check(self.ctx.L.vx_server_drain(self.h), "vx_server_drain")
+
+    def poll(self) -> list[Completion]:
+        out = []
+        buf = (Completion * 1024)()
+        while True:
+            n = C.c_int32()
+            self.ctx.check(self.ctx.L.vx_server_poll(self.h, C.cast(buf, P), 1024, C.byref(n)),
+                           "vx_server_poll")
+            out += [Completion.from_buffer_copy(buf[i]) for i in range(n.value)]
+            if n.value < 1024:
+                return out
+
+    def lookup(self, coord, edge: int):
+        ch, me = P(), P()
+        st = self.ctx.L.vx_server_lookup(self.h, Coord(*coord), edge, C.byref(ch), C.byref(me))
+        if st == ERR_NOT_READY:
+            return None
+        self.ctx.check(st, "vx_server_lookup")
+        return Chunk(self.ctx, ch), ChunkMesh(self.ctx, me, self.formats)
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        self.ctx.check(self.ctx.L.vx_server_stats_get(self.h, C.byref(s)), "vx_server_stats_get")
+        return s
+
+
+def simulate(ctx: Context, *, seed=3, rate=2000.0, duration=2.0, speed=20.0, radius=16,
+             classes=((16, 1.0, 0), (32, 1.0, 0), (64, 1.0, 0)), neighbours=NB_NONE,
+             max_batch=0, max_inflight=0, formats=FMT_VERTICES) -> SimResult:
+    """Streaming simulation (camera random walk + Poisson arrivals) through a
+    Server; classes = ((edge, weight, terrain_id), ...)."""
+    d = SimDesc()
+    d.seed, d.rate_per_s, d.duration_s, d.speed_blocks_per_s = seed, rate, duration, speed
+    d.radius_chunks, d.n_classes, d.neighbours = radius, len(classes), neighbours
+    for i, (e, w, t) in enumerate(classes):
+        d.edges[i], d.weights[i], d.terrain_id[i] = e, w, t
+    d.server = ServerDesc(max_batch, max_inflight, formats, 0)
+    r = SimResult()
+    ctx.check(ctx.L.vx_serve_simulate(ctx.h, C.byref(d), C.byref(r)), "vx_serve_simulate")
+    return r
 
 
 def grid_neighbours(nx: int, nz: int) -> np.ndarray:
