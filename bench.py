@@ -113,44 +113,75 @@ def main():
     n = len(coords)
     log(f"[bench] creating batch: {n} chunks of {EDGE}^3")
     b = V.Batch(ctx, coords, EDGE, tid, nb, formats=V.FMT_VERTICES)
+    b.set_mode(V.BATCH_AUTO)
+    mode = b.mode
     b.run()
     total_faces = b.wait()           # sizes the mesh arena (grows once if needed)
-    log(f"[bench] faces per step: {total_faces}")
-    for _ in range(args.warmup):
-        b.run()
-    ctx.synchronize()
-    # timed region: K graph replays bracketed by device synchronisation
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.run()
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    assert b.wait() == total_faces
+    log(f"[bench] form: {'fused' if mode == V.BATCH_FUSED else 'staged'}; faces per step: {total_faces}")
+
+    def timed(steps, warmup):
+        for _ in range(warmup):
+            b.run()
+        ctx.synchronize()
+        # timed region: K graph replays bracketed by device synchronisation
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run()
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        assert b.wait() == total_faces
+        return el
+
+    elapsed = timed(args.steps, args.warmup)
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * args.steps / elapsed
     log(f"[bench] {ms_per_step:.3f} ms/step, {value:.0f} chunks/s")
+    chosen_ms = b.profile(args.profile_reps)
 
-    # per-kernel HIP-event timing on the context stream
-    stage_ms = b.profile(args.profile_reps)
-    names = ["fill", "count", "scan", "emit"]
-    log("[bench] stage ms: " + ", ".join(f"{k}={v:.4f}" for k, v in zip(names, stage_ms[:4])) +
-        f", graph={stage_ms[4]:.4f}")
     N3, N2 = EDGE ** 3, EDGE ** 2
     fill_bytes = n * (N3 + N3 // 8 + 4 * N2 // 8 + 8)          # ids + occupancy + planes + meta
     fill_flops = n * N3 * t.octaves * K_EVAL
     emit_bytes = n * (N3 // 8 + 4 * N2 // 8 + 4 + 8) + 104 * total_faces
+    fused_bytes = fill_bytes + 104 * total_faces + n * 12       # one pass: writes only
+
+    # staged form for comparison (same outputs, four kernels)
+    staged = None
+    stage_ms = None
+    if b.mode == V.BATCH_FUSED:
+        b.set_mode(V.BATCH_STAGED)
+        b.run()
+        assert b.wait() == total_faces
+        se = timed(max(5, args.steps // 2), 1)
+        staged = {"value": round(n * max(5, args.steps // 2) / se, 1), "unit": "chunks/s",
+                  "ms_per_step": round(se * 1e3 / max(5, args.steps // 2), 4)}
+        stage_ms = b.profile(args.profile_reps)
+        b.set_mode(V.BATCH_AUTO)
+    else:
+        stage_ms = chosen_ms
+    names = ["fill", "count", "scan", "emit"]
+    log("[bench] staged ms: " + ", ".join(f"{k}={v:.4f}" for k, v in zip(names, stage_ms[:4])))
     stages = {
         "fill": {"ms": round(float(stage_ms[0]), 4),
                  "GB/s": round(fill_bytes / (stage_ms[0] * 1e-3) / 1e9, 1),
-                 "TFLOP/s": round(fill_flops / (stage_ms[0] * 1e-3) / 1e12, 2),
-                 "algorithmic_bytes": fill_bytes, "algorithmic_flops": fill_flops},
+                 "TFLOP/s": round(fill_flops / (stage_ms[0] * 1e-3) / 1e12, 2)},
         "count": {"ms": round(float(stage_ms[1]), 4)},
         "scan": {"ms": round(float(stage_ms[2]), 4)},
         "emit": {"ms": round(float(stage_ms[3]), 4),
-                 "GB/s": round(emit_bytes / (stage_ms[3] * 1e-3) / 1e9, 1),
-                 "algorithmic_bytes": emit_bytes},
+                 "GB/s": round(emit_bytes / (stage_ms[3] * 1e-3) / 1e9, 1)},
+        "algorithmic_bytes": {"fill": fill_bytes, "emit": emit_bytes, "fused": fused_bytes},
+        "algorithmic_flops": fill_flops,
     }
-    if stage_ms[0] >= stage_ms[3]:
+    if mode == V.BATCH_FUSED:
+        k_ms = float(chosen_ms[0])
+        tf = fill_flops / (k_ms * 1e-3) / 1e12
+        gbs = fused_bytes / (k_ms * 1e-3) / 1e9
+        log(f"[bench] fused kernel {k_ms:.4f} ms: {tf:.1f} TFLOP/s, {gbs:.0f} GB/s")
+        roof = {"kernel": "fused_kernel<32> (noise+fill+cull+scan+emit)", "bound": "valu",
+                "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "ms": round(k_ms, 4), "hbm_GB/s": round(gbs, 1),
+                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    elif stage_ms[0] >= stage_ms[3]:
         ach = fill_flops / (stage_ms[0] * 1e-3) / 1e12
         roof = {"kernel": "fill_kernel<32> (noise+fill)", "bound": "valu", "achieved": round(ach, 2),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
@@ -203,8 +234,10 @@ def main():
                    "chunks_per_step": n, "edge": EDGE, "faces_per_step": total_faces},
         "roofline": roof,
         "stages": stages,
+        "form": "fused" if mode == V.BATCH_FUSED else "staged",
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "staged_gpu": staged,
     }
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)

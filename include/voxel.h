@@ -199,6 +199,20 @@ typedef struct vx_batch_req {
 int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n, uint32_t formats,
                     uint64_t face_capacity, vx_batch **out);
 int vx_batch_destroy(vx_batch *batch);
+/* Execution form of a batch:
+ *   VX_BATCH_AUTO    fused when eligible, else staged (default)
+ *   VX_BATCH_FUSED   one persistent kernel: noise -> fill -> cull -> compact per
+ *                    chunk, neighbour planes recomputed, output offsets from a
+ *                    single-pass device-wide scan.  Eligible when every in-batch
+ *                    neighbour is the adjacent chunk with the same terrain.
+ *   VX_BATCH_STAGED  four kernels: fill -> count -> scan -> emit
+ * Both forms produce identical bytes. */
+#define VX_BATCH_AUTO 0
+#define VX_BATCH_FUSED 1
+#define VX_BATCH_STAGED 2
+int vx_batch_set_mode(vx_batch *batch, int32_t mode);
+/* The form the next run uses (VX_BATCH_FUSED or VX_BATCH_STAGED). */
+int vx_batch_get_mode(vx_batch *batch, int32_t *mode);
 /* Queues fill -> count -> scan -> emit on the context stream (replayed from a
  * captured hipGraph after the first run).  Asynchronous. */
 int vx_batch_run(vx_ctx *ctx, vx_batch *batch);
@@ -213,8 +227,10 @@ int vx_batch_read_ids(vx_ctx *ctx, vx_batch *batch, int32_t i, uint8_t *dst, uin
  * request order; indices are chunk-local). */
 int vx_batch_read_mesh(vx_ctx *ctx, vx_batch *batch, uint32_t format, void *dst, uint64_t cap);
 int vx_batch_device_buffers(vx_batch *batch, void **instances, void **vertices, void **indices);
-/* Times `reps` replays of each stage with HIP events on the context stream;
- * ms[0..3] = average fill, count, scan, emit kernel time, ms[4] = whole run. */
+/* Times `reps` replays of each stage with HIP events on the context stream.
+ * Staged form: ms[0..3] = average fill, count, scan, emit kernel time;
+ * fused form: ms[0] = fused kernel (with its control reset), ms[1..3] = 0;
+ * ms[4] = whole run replayed from the graph. */
 int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
 /* Plain path for comparison: the same work as one launch per chunk per
  * stage (no batching, no graph), chunk order preserved. */

@@ -158,9 +158,10 @@ def test_generated_chunk_mesh_with_generated_neighbours(ctx, oracle_mod):
 
 # ------------------------------------------------------------------ batches
 def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES,
-                    face_capacity=0, per_chunk=False):
+                    face_capacity=0, per_chunk=False, mode=V.BATCH_AUTO):
     tids = [ctx.terrain(t) for t in terrains]
     b = V.Batch(ctx, coords, edges, tids, nb, formats=formats, face_capacity=face_capacity)
+    b.set_mode(mode)
     if per_chunk:
         b.run_per_chunk()
     else:
@@ -188,23 +189,53 @@ def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES 
     return b
 
 
-def test_batch_config1_grid(ctx, oracle_mod):
+MODES = [V.BATCH_FUSED, V.BATCH_STAGED]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_config1_grid(ctx, oracle_mod, mode):
     """Config 1: 4x4 grid of 16^3 heightmap chunks, all in-grid neighbours."""
-    say("batch C1")
+    say("batch C1", mode)
     coords = [(x, 0, z) for x in range(4) for z in range(4)]
-    batch_vs_oracle(ctx, oracle_mod, coords, [16] * 16, [C1] * 16, V.grid_neighbours(4, 4))
+    batch_vs_oracle(ctx, oracle_mod, coords, [16] * 16, [C1] * 16, V.grid_neighbours(4, 4),
+                    mode=mode)
 
 
-@pytest.mark.parametrize("per_chunk", [False, True])
-def test_batch_config2_subgrid(ctx, oracle_mod, per_chunk):
+@pytest.mark.parametrize("mode,per_chunk", [(V.BATCH_FUSED, False), (V.BATCH_STAGED, False),
+                                            (V.BATCH_STAGED, True)])
+def test_batch_config2_subgrid(ctx, oracle_mod, mode, per_chunk):
     """Config 2 terrain on an 8x8 sub-grid of 32^3 chunks."""
-    say("batch C2 8x8", per_chunk)
+    say("batch C2 8x8", mode, per_chunk)
     coords = [(x + 20, 0, z + 30) for x in range(8) for z in range(8)]
     batch_vs_oracle(ctx, oracle_mod, coords, [32] * 64, [C2] * 64, V.grid_neighbours(8, 8),
-                    per_chunk=per_chunk)
+                    per_chunk=per_chunk, mode=mode)
 
 
-def test_batch_ragged(ctx, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("edge", [16, 64])
+def test_batch_density_grid_other_edges(ctx, oracle_mod, mode, edge):
+    """3-D density with in-grid neighbours at edges 16 and 64 (halo lines of
+    every width), including negative coordinates."""
+    say("batch density grid", mode, edge)
+    coords = [(x - 2, -1 if edge == 16 else 0, z - 1) for x in range(3) for z in range(3)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [edge] * 9, [C5] * 9, V.grid_neighbours(3, 3),
+                    mode=mode)
+
+
+def test_fused_is_refused_for_non_adjacent_neighbours(ctx, oracle_mod):
+    """A neighbour index that is not the adjacent chunk cannot be recomputed:
+    the batch runs staged and still matches the oracle."""
+    coords = [(0, 0, 0), (5, 0, 5), (9, 0, 1)]
+    nb = [[-1, 1, -1, 2], [0, -1, -1, -1], [-1, -1, 0, -1]]
+    b = V.Batch(ctx, coords, 32, ctx.terrain(C2), nb)
+    assert b.mode == V.BATCH_STAGED
+    with pytest.raises(V.VoxelError):
+        b.set_mode(V.BATCH_FUSED)
+    batch_vs_oracle(ctx, oracle_mod, coords, [32] * 3, [C2] * 3, nb)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_ragged(ctx, oracle_mod, mode):
     """Config 3 shape: edges from {16,32,64}, 30% air / 10% solid / 60%
     heightmap surface (seed 11), no neighbours, longest-first schedule."""
     say("batch ragged")
@@ -222,20 +253,23 @@ def test_batch_ragged(ctx, oracle_mod):
             terrains.append(V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=11, octaves=4,
                                       frequency=1 / 32, height_scale=float(e), height_max=int(e) - 1))
     coords = [(i % 8, 0, i // 8) for i in range(n)]
-    batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1))
+    batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1), mode=mode)
 
 
-def test_batch_overflow_grows_arena(ctx, oracle_mod):
-    say("batch overflow")
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_overflow_grows_arena(ctx, oracle_mod, mode):
+    say("batch overflow", mode)
     coords = [(x, 0, z) for x in range(3) for z in range(3)]
     batch_vs_oracle(ctx, oracle_mod, coords, [32] * 9, [C2] * 9, V.grid_neighbours(3, 3),
-                    face_capacity=100)
+                    face_capacity=100, mode=mode)
 
 
-def test_batch_replay_is_stable(ctx, oracle_mod):
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_replay_is_stable(ctx, oracle_mod, mode):
     """hipGraph replays give identical results, with eager work between."""
     coords = [(x, 0, z) for x in range(4) for z in range(4)]
     b = V.Batch(ctx, coords, 32, ctx.terrain(C2), V.grid_neighbours(4, 4))
+    b.set_mode(mode)
     b.run()
     t0 = b.wait()
     v0 = b.mesh(V.FMT_VERTICES).copy()

@@ -110,7 +110,18 @@ hipError_t enqueue_emit(vx_batch *b) {
                        b->ctx->tables_dev, b->overflow, b->ctx->stream);
 }
 
+bool use_fused(const vx_batch *b) {
+    return b->fused_ok && b->mode != VX_BATCH_STAGED;
+}
+
+hipError_t enqueue_fused(vx_batch *b) {
+    vx_ctx *ctx = b->ctx;
+    return launch_fused(b->fused_dev, b->n, b->max_edge, ctx->terrains_dev, ctx->lut_dev,
+                        ctx->tables_dev, b->fctl_ptrs, b->out, ctx->stream);
+}
+
 hipError_t enqueue_all(vx_batch *b) {
+    if (use_fused(b)) return enqueue_fused(b);
     hipError_t e = enqueue_fill(b);
     if (e == hipSuccess) e = enqueue_count(b);
     if (e == hipSuccess) e = enqueue_scan(b);
@@ -152,6 +163,8 @@ void free_batch(vx_batch *b) {
     ctx->dev_free(b->order_dev);
     ctx->dev_free(b->ctl);
     ctx->dev_free(b->arena);
+    ctx->dev_free(b->fused_dev);
+    ctx->dev_free(b->fctl);
     delete b;
 }
 
@@ -261,6 +274,43 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
                              : nullptr;
         }
     }
+    // fused form: every in-batch neighbour must be the adjacent chunk with the
+    // same terrain, so its facing plane can be recomputed from noise
+    static const int ddx[4] = {-1, 1, 0, 0}, ddz[4] = {0, 0, -1, 1};
+    b->fused_ok = true;
+    for (int i = 0; i < n && b->fused_ok; i++)
+        for (int k = 0; k < 4; k++) {
+            const int j = reqs[i].nb[k];
+            if (j < 0) continue;
+            const vx_batch_req &a = reqs[i], &c = reqs[j];
+            if (c.coord.x != a.coord.x + ddx[k] || c.coord.y != a.coord.y ||
+                c.coord.z != a.coord.z + ddz[k] || c.terrain_id != a.terrain_id) {
+                b->fused_ok = false;
+                break;
+            }
+        }
+    std::vector<FusedWork> fu;
+    if (b->fused_ok) {
+        fu.resize(n);
+        for (int i = 0; i < n; i++) {
+            fu[i].f = fw[i];
+            fu[i].m = mw[i];
+            for (int k = 0; k < 4; k++)
+                fu[i].halo[k] = reqs[i].nb[k] >= 0 ? kHaloRecompute : kHaloAbsent;
+        }
+        if ((st = ctx->dev_alloc((size_t)n * sizeof(FusedWork), (void **)&b->fused_dev)))
+            return fail(st);
+        if ((st = ctx->dev_alloc(256 + (size_t)n * 8, &b->fctl))) return fail(st);
+        b->fctl_ptrs.ticket = (uint32_t *)b->fctl;
+        b->fctl_ptrs.error = (uint32_t *)b->fctl + 1;
+        b->fctl_ptrs.status = (uint64_t *)((char *)b->fctl + 256);
+        b->fctl_ptrs.counts = b->counts;
+        b->fctl_ptrs.bases = b->bases;
+        b->fctl_ptrs.total = b->total;
+        b->fctl_ptrs.overflow = b->overflow;
+        VX_HIP(ctx, hipMemcpyAsync(b->fused_dev, fu.data(), fu.size() * sizeof(FusedWork),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    }
     VX_HIP(ctx, hipMemcpyAsync(b->fill_dev, fw.data(), fw.size() * sizeof(FillWork),
                                hipMemcpyHostToDevice, ctx->stream));
     VX_HIP(ctx, hipMemcpyAsync(b->mesh_dev, mw.data(), mw.size() * sizeof(MeshWork),
@@ -289,6 +339,24 @@ extern "C" int vx_batch_destroy(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     vx_batch_unref(b);
+    return VX_OK;
+}
+
+extern "C" int vx_batch_set_mode(vx_batch *b, int32_t mode) {
+    if (!b || mode < VX_BATCH_AUTO || mode > VX_BATCH_STAGED) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    if (mode == VX_BATCH_FUSED && !b->fused_ok) return VX_ERR_INVALID;
+    if (mode != b->mode) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        drop_graph(b);
+        b->mode = mode;
+    }
+    return VX_OK;
+}
+
+extern "C" int vx_batch_get_mode(vx_batch *b, int32_t *mode) {
+    if (!b || !mode) return VX_ERR_INVALID;
+    *mode = use_fused(b) ? VX_BATCH_FUSED : VX_BATCH_STAGED;
     return VX_OK;
 }
 
@@ -330,10 +398,21 @@ extern "C" int vx_batch_wait(vx_ctx *ctx, vx_batch *b, uint64_t *total_faces) {
     VX_HIP(ctx, hipMemcpyAsync(ctx->pinned, b->total, 8, hipMemcpyDeviceToHost, ctx->stream));
     VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
     VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t ferr = 0;
+    if (use_fused(b)) {
+        VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 16, b->fctl_ptrs.error, 4, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy(&ferr, ctx->pinned + 16, 4);
+    }
     uint64_t total;
     uint32_t overflow;
     memcpy(&total, ctx->pinned, 8);
     memcpy(&overflow, ctx->pinned + 8, 4);
+    if (ferr) {
+        ctx->last_error = "fused batch: a look-back wait exceeded its bound";
+        return VX_ERR_DEVICE;
+    }
     if (overflow || total > b->capacity) {
         // grow the arena to the exact total and re-run only the emit stage;
         // counts and bases from this run stay valid
@@ -419,7 +498,21 @@ extern "C" int vx_batch_profile(vx_ctx *ctx, vx_batch *b, int32_t reps, float ms
     for (int i = 0; i < 6; i++) VX_HIP(ctx, hipEventCreate(&ev[i]));
     double acc[5] = {0, 0, 0, 0, 0};
     int st = VX_OK;
-    for (int r = 0; r < reps && !st; r++) {
+    for (int r = 0; r < reps && !st && use_fused(b); r++) {
+        hipError_t e = hipEventRecord(ev[0], ctx->stream);
+        if (e == hipSuccess) e = enqueue_fused(b);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        if (e != hipSuccess) {
+            ctx->set_error(e, "vx_batch_profile");
+            st = VX_ERR_DEVICE;
+            break;
+        }
+        float t = 0;
+        (void)hipEventElapsedTime(&t, ev[0], ev[1]);
+        acc[0] += t;
+    }
+    for (int r = 0; r < reps && !st && !use_fused(b); r++) {
         hipError_t e = hipEventRecord(ev[0], ctx->stream);
         if (e == hipSuccess) e = enqueue_fill(b);
         if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);

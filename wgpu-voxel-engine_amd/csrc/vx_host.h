@@ -117,6 +117,11 @@ struct vx_batch {
     uint64_t capacity = 0;          // faces the mesh arena holds
     void *arena = nullptr;
     vx::MeshOut out{};
+    int32_t mode = 0;               // VX_BATCH_* requested
+    bool fused_ok = false;          // fused form eligible
+    vx::FusedWork *fused_dev = nullptr;
+    void *fctl = nullptr;           // ticket | error | status[n]
+    vx::FusedCtl fctl_ptrs{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool ran = false;

@@ -91,6 +91,31 @@ struct MeshOut {
     uint32_t pad0;
 };
 
+// Fused noise -> cull -> compact: per-side source of the neighbour plane.
+enum : uint32_t { kHaloAbsent = 0u, kHaloResident = 1u, kHaloRecompute = 2u };
+
+// One chunk of the fused kernel: its fill outputs, its mesh description
+// (m.nb[k] read for resident sides) and how each neighbour plane is obtained.
+// kHaloRecompute is only used when that neighbour is the adjacent chunk of
+// the same batch with the same terrain and edge, so the recomputed plane is
+// bit-identical to the one the neighbour writes itself.
+struct FusedWork {
+    FillWork f;
+    MeshWork m;
+    uint32_t halo[4];
+};
+
+// Control words of one fused run; zeroed by the reset kernel.
+struct FusedCtl {
+    uint32_t *ticket;    // next chunk to hand out (processing order = output order)
+    uint64_t *status;    // per chunk: flag(2 bits) | face count or inclusive prefix
+    uint32_t *counts;    // per chunk face count (for the host)
+    uint64_t *bases;     // per chunk first face (for the host)
+    uint64_t *total;     // faces of the whole batch
+    uint32_t *overflow;  // set when a chunk did not fit the arena
+    uint32_t *error;     // set when a look-back wait exceeded its bound
+};
+
 // Six frustum planes [near far left right top bottom] x (a b c d), passed
 // to the cull kernel by value.
 struct Planes {
@@ -115,6 +140,9 @@ hipError_t launch_emit_one(const MeshWork &w, uint64_t base, uint32_t faces, Mes
 hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const uint64_t *bases, const uint32_t *counts, MeshOut out,
                        const MeshTables *tables, uint32_t *overflow, hipStream_t s);
+hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
+                        const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
+                        FusedCtl ctl, MeshOut out, hipStream_t s);
 hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n, int32_t edge,
                           uint8_t *mask, hipStream_t s);
 

@@ -84,52 +84,105 @@ __device__ float noise2(const int *p, float x, float y) {
 }
 
 // --------------------------------------------------------------- fill
+// Occupancy rows live in LDS as RowType<MAXN> (u64 when MAXN = 64), i.e. at
+// a stride of RS u32 words; the boundary planes of the four neighbours the
+// same way.  The fill, the mesh and the fused kernel share this layout.
+template <int MAXN> struct RowStride { static constexpr int v = MAXN == 64 ? 2 : 1; };
+
 // LDS layout of one fill workgroup, sized for the largest edge it handles.
 template <int MAXN> struct FillShared {
     int perm[512];
     float4 lut[8][16];
     float4 ytab[kMaxOctaves * MAXN];   // per (octave, y): fy, fy-1, fade(fy), Y cell
-    uint32_t occ[MAXN * MAXN * (MAXN == 64 ? 2 : 1)];
+    typename RowType<MAXN>::T occ[MAXN * MAXN];
+    typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t red[kBlock / 64];
 };
 
-// Writes the occupancy row(s) produced by one ballot: for N <= 32 the wave
-// covers 64/N consecutive z rows at one y; for N = 64 one row of 2 words.
+// Columns are evaluated in "lines" of N lanes.  Lines 0..N-1 are the chunk's
+// own rows (columns x = 0..N-1 at z = line); lines N..N+n-1 are recomputed
+// neighbour planes: side -x (world x = ox-1, lanes along z), +x (x = ox+N),
+// -z (world z = oz-1, lanes along x), +z (z = oz+N).  A ballot over a wave
+// then yields occupancy rows or plane words directly.
+struct Halo {
+    int n;
+    uint32_t sides;  // side of halo line i in bits [2i, 2i+2)
+    __device__ __forceinline__ int side(int i) const { return (int)((sides >> (2 * i)) & 3u); }
+};
+
 template <int N>
-__device__ __forceinline__ void store_ballot_rows(uint32_t *s_occ, uint64_t b, int y, int z0,
-                                                  int lane) {
+__device__ __forceinline__ void line_column(int li, int lane, const Halo &H, int ox, int oz,
+                                            int &wx, int &wz) {
+    const int p = lane % N;
+    if (li < N) {
+        wx = ox + p;
+        wz = oz + li;
+        return;
+    }
+    const int k = H.side(min(li - N, 3));
+    if (k == 0) { wx = ox - 1; wz = oz + p; }
+    else if (k == 1) { wx = ox + N; wz = oz + p; }
+    else if (k == 2) { wx = ox + p; wz = oz - 1; }
+    else { wx = ox + p; wz = oz + N; }
+}
+
+// Stores the bits of one ballot taken at height y by line group g.
+template <int N, int MAXN, class SH>
+__device__ __forceinline__ void store_lines(SH &S, const Halo &H, uint64_t b, int y, int g,
+                                            int lane) {
+    constexpr int RS = RowStride<MAXN>::v;
+    uint32_t *occ = reinterpret_cast<uint32_t *>(S.occ);
     if (N == 64) {
-        if (lane < 2) s_occ[(y * N + z0) * 2 + lane] = lane ? (uint32_t)(b >> 32) : (uint32_t)b;
+        if (lane < 2) {
+            const uint32_t v = lane ? (uint32_t)(b >> 32) : (uint32_t)b;
+            const int li = g;
+            if (li < N) occ[(y * N + li) * 2 + lane] = v;
+            else if (li - N < H.n) reinterpret_cast<uint32_t *>(S.nb[H.side(li - N)])[y * 2 + lane] = v;
+        }
     } else {
-        constexpr int ZPW = 64 / N;
+        constexpr int L = 64 / N;
         constexpr uint32_t MASK = N >= 32 ? 0xFFFFFFFFu : ((1u << (N & 31)) - 1u);
-        if (lane < ZPW) s_occ[y * N + z0 + lane] = (uint32_t)(b >> (lane * N)) & MASK;
+        if (lane < L) {
+            const int li = g * L + lane;
+            const uint32_t v = (uint32_t)(b >> (lane * N)) & MASK;
+            uint32_t *dst = nullptr;
+            if (li < N) dst = occ + (y * N + li) * RS;
+            else if (li - N < H.n) dst = reinterpret_cast<uint32_t *>(S.nb[H.side(li - N)]) + y * RS;
+            if (dst) {
+                dst[0] = v;
+                if (RS == 2) dst[1] = 0u;
+            }
+        }
     }
 }
 
-template <int N, int MAXN>
-__device__ void fill_rows_density(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
-    constexpr int ZPW = 64 / N;
-    constexpr int PASSES = N * N / kBlock;
+// 3-D fBm density for every line: the column's x/z lattice terms are hoisted
+// per octave, the y terms come from S.ytab, the 8 corner gradients are
+// rehashed only when the y lattice cell changes.  Voxel is solid iff
+// fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0 (oracle: vxo_chunk_fill).
+template <int N, int MAXN, bool DBG, class SH>
+__device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
+                              float *noise_out) {
+    constexpr int L = 64 / N;
     constexpr int SEG = N < 32 ? N : 32;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x = lane % N;
-    const int ox = w.cx * N, oy = w.cy * N, oz = w.cz * N;
+    const int G = (N + H.n + L - 1) / L;
     const int octaves = T.octaves;
-    for (int p = 0; p < PASSES; p++) {
-        const int z0 = (p * (kBlock / 64) + wave) * ZPW;
-        const int z = z0 + lane / N;
+    for (int g = wave; g < G; g += kBlock / 64) {
+        const int li = g * L + lane / N;
+        int wxi, wzi;
+        line_column<N>(li, lane, H, ox, oz, wxi, wzi);
         for (int y0 = 0; y0 < N; y0 += SEG) {
             float sum[SEG];
 #pragma unroll
             for (int i = 0; i < SEG; i++) sum[i] = 0.0f;
             float f = T.frequency, amp = 1.0f;
             for (int o = 0; o < octaves; o++) {
-                const float X = (float)(ox + x) * f;
+                const float X = (float)wxi * f;
                 const float fxl = floorf(X);
                 const float fx = X - fxl, fx1 = fx - 1.0f, u = fade(fx);
                 const int Xi = (int)fxl & 255;
-                const float Z = (float)(oz + z) * f;
+                const float Z = (float)wzi * f;
                 const float fzl = floorf(Z);
                 const float fz = Z - fzl, fz1 = fz - 1.0f, wz = fade(fz);
                 const int Zi = (int)fzl & 255;
@@ -154,11 +207,11 @@ __device__ void fill_rows_density(const FillWork &w, const TerrainDev &T, FillSh
                                           S.perm[AB + 1], S.perm[BB + 1]};
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
-                            const float4 L = S.lut[k][h[k] & 15];
+                            const float4 Lk = S.lut[k][h[k] & 15];
                             const float xr = (k & 1) ? fx1 : fx;
                             const float zr = (k & 4) ? fz1 : fz;
-                            c[k] = fmaf(L.x, xr, L.y * zr);
-                            gy[k] = L.z;
+                            c[k] = fmaf(Lk.x, xr, Lk.y * zr);
+                            gy[k] = Lk.z;
                         }
                     }
                     const float g0 = fmaf(t.x, gy[0], c[0]), g1 = fmaf(t.x, gy[1], c[1]);
@@ -172,29 +225,32 @@ __device__ void fill_rows_density(const FillWork &w, const TerrainDev &T, FillSh
                 f = f * T.lacunarity;
                 amp = amp * T.gain;
             }
-#pragma unroll
+#pragma unroll 1
             for (int yy = 0; yy < SEG; yy++) {
                 const int y = y0 + yy;
                 const float thr = (float)(oy + y) * T.y_scale;
                 const bool solid = (sum[yy] - thr) > 0.0f;
-                store_ballot_rows<N>(S.occ, __ballot(solid), y, z0, lane);
-                if (w.noise_out) w.noise_out[(size_t)x + (size_t)N * z + (size_t)N * N * y] = sum[yy];
+                store_lines<N, MAXN>(S, H, __ballot(solid), y, g, lane);
+                if (DBG && li < N)
+                    noise_out[(size_t)(wxi - ox) + (size_t)N * li + (size_t)N * N * y] = sum[yy];
             }
         }
     }
 }
 
-template <int N, int MAXN>
-__device__ void fill_rows_heightmap(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
-    constexpr int ZPW = 64 / N;
-    constexpr int PASSES = N * N / kBlock;
+// 2-D fBm heightmap for every line: h = floor(height_scale*(n+1)/2) clamped
+// to [0, height_max]; voxel solid iff wy <= h.
+template <int N, int MAXN, bool DBG, class SH>
+__device__ void heightmap_lines(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
+                                SH &S, float *noise_out) {
+    constexpr int L = 64 / N;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x = lane % N;
-    const int ox = w.cx * N, oy = w.cy * N, oz = w.cz * N;
-    for (int p = 0; p < PASSES; p++) {
-        const int z0 = (p * (kBlock / 64) + wave) * ZPW;
-        const int z = z0 + lane / N;
-        const float wx = (float)(ox + x), wzf = (float)(oz + z);
+    const int G = (N + H.n + L - 1) / L;
+    for (int g = wave; g < G; g += kBlock / 64) {
+        const int li = g * L + lane / N;
+        int wxi, wzi;
+        line_column<N>(li, lane, H, ox, oz, wxi, wzi);
+        const float wx = (float)wxi, wzf = (float)wzi;
         float sum = 0.0f, amp = 1.0f, f = T.frequency;
         for (int o = 0; o < T.octaves; o++) {
             sum = sum + amp * noise2(S.perm, wx * f, wzf * f);
@@ -206,23 +262,59 @@ __device__ void fill_rows_heightmap(const FillWork &w, const TerrainDev &T, Fill
         if (!(hf >= 0.0f)) h = 0;
         else if (hf > (float)T.height_max) h = T.height_max;
         else h = (int)hf;
-        if (w.noise_out) w.noise_out[(size_t)x + (size_t)N * z] = sum;
-        for (int y = 0; y < N; y++) store_ballot_rows<N>(S.occ, __ballot(oy + y <= h), y, z0, lane);
+        if (DBG && li < N) noise_out[(size_t)(wxi - ox) + (size_t)N * li] = sum;
+        for (int y = 0; y < N; y++) store_lines<N, MAXN>(S, H, __ballot(oy + y <= h), y, g, lane);
+    }
+}
+
+// Per-(octave, y) lattice terms of the chunk's world rows.
+template <int N, class SH>
+__device__ __forceinline__ void build_ytab(const TerrainDev &T, int oy, SH &S) {
+    for (int i = threadIdx.x; i < T.octaves * N; i += kBlock) {
+        const int o = i / N, y = i - o * N;
+        float f = T.frequency;
+        for (int k = 0; k < o; k++) f = f * T.lacunarity;
+        const float Y = (float)(oy + y) * f;
+        const float fyl = floorf(Y);
+        const float fy = Y - fyl;
+        S.ytab[i] = make_float4(fy, fy - 1.0f, fade(fy), __int_as_float((int)fyl & 255));
+    }
+}
+
+// Fills S.occ (and the recomputed halo planes) for terrain T.
+template <int N, int MAXN, bool DBG, class SH>
+__device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
+                              float *noise_out) {
+    using R = typename RowType<MAXN>::T;
+    if (T.kind == kDensity) {
+        build_ytab<N>(T, oy, S);
+        __syncthreads();
+        density_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
+    } else if (T.kind == kHeightmap) {
+        heightmap_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
+    } else {
+        const R full = T.kind == kSolid ? (N == 64 ? (R)~(R)0 : (R)((N >= 32 ? 0xFFFFFFFFu
+                                                                             : ((1u << (N & 31)) - 1u))))
+                                        : (R)0;
+        for (int i = threadIdx.x; i < N * N; i += kBlock) S.occ[i] = full;
+        for (int i = threadIdx.x; i < H.n * N; i += kBlock) S.nb[H.side(i / N)][i % N] = full;
     }
 }
 
 // Writes a chunk whose occupancy rows are complete in S.occ: occupancy and
 // boundary planes always, u8 ids (all solid voxels = kBlockGrass) when
 // write_ids, and meta = {solid block id, solid voxel count}.
-template <int N, int MAXN>
-__device__ void write_chunk_outputs(const FillWork &w, FillShared<MAXN> &S, bool write_ids,
-                                    uint32_t solid_type) {
+template <int N, int MAXN, class SH>
+__device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, uint32_t solid_type) {
     constexpr int W = N == 64 ? 2 : 1;
+    constexpr int RS = RowStride<MAXN>::v;
     constexpr int ROWS = N * N;
+    const uint32_t *occ = reinterpret_cast<const uint32_t *>(S.occ);
     const int tid = threadIdx.x;
     uint32_t solid = 0;
     for (int i = tid; i < ROWS * W; i += kBlock) {
-        const uint32_t v = S.occ[i];
+        const int r = i / W, j = i - r * W;
+        const uint32_t v = occ[r * RS + j];
         w.occ[i] = v;
         solid += __popc(v);
     }
@@ -235,12 +327,12 @@ __device__ void write_chunk_outputs(const FillWork &w, FillShared<MAXN> &S, bool
             const int zlim = N < 32 ? N : 32;
             for (int zz = 0; zz < zlim; zz++) {
                 const int z = j * 32 + zz;
-                const uint32_t word = S.occ[(y * N + z) * W + (xb >> 5)];
+                const uint32_t word = occ[(y * N + z) * RS + (xb >> 5)];
                 v |= ((word >> (xb & 31)) & 1u) << zz;
             }
         } else {
             const int z = pl == 2 ? 0 : N - 1;
-            v = S.occ[(y * N + z) * W + j];
+            v = occ[(y * N + z) * RS + j];
         }
         w.planes[i] = v;
     }
@@ -250,7 +342,7 @@ __device__ void write_chunk_outputs(const FillWork &w, FillShared<MAXN> &S, bool
         uint4 *ids4 = reinterpret_cast<uint4 *>(w.ids);
         for (int q = tid; q < ROWS * PER_ROW; q += kBlock) {
             const int r = q / PER_ROW, part = q - r * PER_ROW;
-            const uint32_t word = S.occ[r * W + (part >> 1)];
+            const uint32_t word = occ[r * RS + (part >> 1)];
             const uint32_t bits = (word >> ((part & 1) * 16)) & 0xFFFFu;
             uint32_t o[4];
 #pragma unroll
@@ -268,29 +360,10 @@ __device__ void write_chunk_outputs(const FillWork &w, FillShared<MAXN> &S, bool
     }
 }
 
-template <int N, int MAXN>
+template <int N, int MAXN, bool DBG>
 __device__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
-    constexpr int W = N == 64 ? 2 : 1;
-    constexpr int ROWS = N * N;
-    const int tid = threadIdx.x;
-    if (T.kind == kDensity) {
-        for (int i = tid; i < T.octaves * N; i += kBlock) {
-            const int o = i / N, y = i - o * N;
-            float f = T.frequency;
-            for (int k = 0; k < o; k++) f = f * T.lacunarity;
-            const float Y = (float)(w.cy * N + y) * f;
-            const float fyl = floorf(Y);
-            const float fy = Y - fyl;
-            S.ytab[i] = make_float4(fy, fy - 1.0f, fade(fy), __int_as_float((int)fyl & 255));
-        }
-        __syncthreads();
-        fill_rows_density<N, MAXN>(w, T, S);
-    } else if (T.kind == kHeightmap) {
-        fill_rows_heightmap<N, MAXN>(w, T, S);
-    } else {
-        const uint32_t full = T.kind == kSolid ? (N >= 32 ? 0xFFFFFFFFu : ((1u << (N & 31)) - 1u)) : 0u;
-        for (int i = tid; i < ROWS * W; i += kBlock) S.occ[i] = full;
-    }
+    const Halo H{0, 0u};
+    generate_rows<N, MAXN, DBG>(T, w.cx * N, w.cy * N, w.cz * N, H, S, w.noise_out);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
 }
@@ -300,9 +373,11 @@ __device__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MA
 template <int N, int MAXN>
 __device__ void pack_chunk(const FillWork &w, FillShared<MAXN> &S) {
     constexpr int W = N == 64 ? 2 : 1;
+    constexpr int RS = RowStride<MAXN>::v;
     __shared__ uint32_t s_types;
     if (threadIdx.x == 0) s_types = 0;
     __syncthreads();
+    uint32_t *occ = reinterpret_cast<uint32_t *>(S.occ);
     uint32_t types = 0;
     for (int i = threadIdx.x; i < N * N * W; i += kBlock) {
         const int r = i / W, j = i - r * W;
@@ -314,7 +389,8 @@ __device__ void pack_chunk(const FillWork &w, FillShared<MAXN> &S) {
             if (b != kBlockAir) bits |= 1u << (x - xs);
             types |= 1u << (b & 7);
         }
-        S.occ[i] = bits;
+        occ[r * RS + j] = bits;
+        if (RS == 2 && W == 1) occ[r * RS + 1] = 0u;
     }
     atomicOr(&s_types, types);
     __syncthreads();
@@ -332,10 +408,19 @@ __device__ __forceinline__ void fill_item(const FillWork &w, const TerrainDev *t
     for (int i = threadIdx.x; i < 512; i += kBlock) S.perm[i] = T.perm[i];
     for (int i = threadIdx.x; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
     __syncthreads();
+    if (w.noise_out) {  // parity-check path: also stores the raw noise
+        switch (w.edge) {
+        case 16: fill_chunk<16, MAXN, true>(w, T, S); break;
+        case 32: if (MAXN >= 32) fill_chunk<(MAXN >= 32 ? 32 : 16), MAXN, true>(w, T, S); break;
+        case 64: if (MAXN >= 64) fill_chunk<(MAXN >= 64 ? 64 : 16), MAXN, true>(w, T, S); break;
+        default: break;
+        }
+        return;
+    }
     switch (w.edge) {
-    case 16: fill_chunk<16, MAXN>(w, T, S); break;
-    case 32: if (MAXN >= 32) fill_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, T, S); break;
-    case 64: if (MAXN >= 64) fill_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(w, T, S); break;
+    case 16: fill_chunk<16, MAXN, false>(w, T, S); break;
+    case 32: if (MAXN >= 32) fill_chunk<(MAXN >= 32 ? 32 : 16), MAXN, false>(w, T, S); break;
+    case 64: if (MAXN >= 64) fill_chunk<(MAXN >= 64 ? 64 : 16), MAXN, false>(w, T, S); break;
     default: break;
     }
 }
@@ -380,8 +465,8 @@ template <int MAXN> struct MeshShared {
     uint32_t red[kBlock / 64];
 };
 
-template <int N, int MAXN>
-__device__ __forceinline__ void load_rows(const MeshWork &w, MeshShared<MAXN> &S) {
+template <int N, int MAXN, class SH>
+__device__ __forceinline__ void load_rows(const MeshWork &w, SH &S) {
     using R = typename RowType<MAXN>::T;
     constexpr int W = N == 64 ? 2 : 1;
     const int tid = threadIdx.x;
@@ -405,8 +490,8 @@ __device__ __forceinline__ void load_rows(const MeshWork &w, MeshShared<MAXN> &S
 // Six face masks of row (y, z), bit x set iff voxel x emits that face
 // (chunk.rs:77-240): neighbour cell air, or outside the chunk with the
 // neighbour chunk absent, or outside vertically.
-template <int N, int MAXN>
-__device__ __forceinline__ void face_masks(const MeshShared<MAXN> &S, int y, int z,
+template <int N, int MAXN, class SH>
+__device__ __forceinline__ void face_masks(const SH &S, int y, int z,
                                            typename RowType<MAXN>::T m[6]) {
     using R = typename RowType<MAXN>::T;
     const R self = S.occ[y * N + z];
@@ -424,11 +509,10 @@ __device__ __forceinline__ void face_masks(const MeshShared<MAXN> &S, int y, int
     m[5] = self & ~up;
 }
 
-template <int N, int MAXN>
-__device__ void count_chunk(const MeshWork &w, MeshShared<MAXN> &S, uint32_t *count_out) {
+// Visible faces of the chunk whose rows and neighbour planes are in LDS.
+template <int N, int MAXN, class SH>
+__device__ uint32_t count_rows(SH &S) {
     using R = typename RowType<MAXN>::T;
-    load_rows<N, MAXN>(w, S);
-    __syncthreads();
     uint32_t c = 0;
     for (int r = threadIdx.x; r < N * N; r += kBlock) {
         R m[6];
@@ -436,7 +520,14 @@ __device__ void count_chunk(const MeshWork &w, MeshShared<MAXN> &S, uint32_t *co
 #pragma unroll
         for (int d = 0; d < 6; d++) c += popc(m[d]);
     }
-    const uint32_t total = block_sum(c, S.red);
+    return block_sum(c, S.red);
+}
+
+template <int N, int MAXN>
+__device__ void count_chunk(const MeshWork &w, MeshShared<MAXN> &S, uint32_t *count_out) {
+    load_rows<N, MAXN>(w, S);
+    __syncthreads();
+    const uint32_t total = count_rows<N, MAXN>(S);
     if (threadIdx.x == 0) *count_out = total;
 }
 
@@ -504,14 +595,12 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
 // face order is (y, z, x, direction) exactly as chunk.rs:65-240 pushes them.
 // Faces are staged as 32-bit descriptors (row | x << 12 | dir << 18) in LDS
 // tiles of kDescTile, then one lane per face writes its records.
-template <int N, int MAXN>
-__device__ void emit_chunk(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
-                           MeshShared<MAXN> &S) {
+template <int N, int MAXN, class SH>
+__device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
+                          SH &S) {
     using R = typename RowType<MAXN>::T;
     constexpr int RPT = (N * N) / kBlock;
     const int tid = threadIdx.x;
-    load_rows<N, MAXN>(w, S);
-    __syncthreads();
     const int r0 = tid * RPT;
     uint32_t rc[RPT];
     uint32_t mine = 0;
@@ -617,6 +706,14 @@ __device__ void emit_chunk(const MeshWork &w, uint64_t base, uint32_t F, const M
     }
 }
 
+template <int N, int MAXN>
+__device__ void emit_chunk(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
+                           MeshShared<MAXN> &S) {
+    load_rows<N, MAXN>(w, S);
+    __syncthreads();
+    emit_rows<N, MAXN>(w, base, F, out, S);
+}
+
 template <int MAXN>
 __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint32_t F,
                                           const MeshOut &out, const MeshTables *tables,
@@ -656,6 +753,159 @@ __global__ void __launch_bounds__(kBlock) emit_one_kernel(MeshWork w, uint64_t b
                                                           uint32_t *__restrict__ overflow) {
     __shared__ MeshShared<MAXN> S;
     emit_item<MAXN>(w, base, F, out, tables, overflow, S);
+}
+
+// --------------------------------------------------------------- fused
+// One persistent kernel runs noise -> fill -> cull -> compact for a whole
+// batch.  Workgroups take chunks in batch order from a ticket counter; each
+// chunk recomputes the neighbour planes it needs (a 4N-column halo, 12.5% of
+// the noise work at N = 32) instead of waiting for its neighbours, counts
+// its faces, and obtains its first output face with a decoupled look-back
+// over the chunks before it (Merrill & Garland, "Single-pass Parallel Prefix
+// Scan with Decoupled Look-back", 2016).  A workgroup only waits on tickets
+// handed out before its own, i.e. on running workgroups, so the wait always
+// ends; it is also bounded (kSpinLimit) and reports through ctl.error.
+template <int MAXN> struct FusedShared {
+    int perm[512];
+    float4 lut[8][16];
+    float4 ytab[kMaxOctaves * MAXN];
+    typename RowType<MAXN>::T occ[MAXN * MAXN];
+    typename RowType<MAXN>::T nb[4][MAXN];
+    uint32_t desc[kDescTile];
+    float P[6][2][4][3];
+    float Mb[6][2][3][3];
+    uint32_t tex[3][6];
+    uint32_t red[kBlock / 64];
+    int item;
+    int terrain;
+    uint64_t base;
+};
+
+constexpr uint64_t kFlagAgg = 1ull << 62, kFlagIncl = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int N, int MAXN>
+__device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int n,
+                            const FusedCtl &ctl, const MeshOut &out, FusedShared<MAXN> &S) {
+    using R = typename RowType<MAXN>::T;
+    constexpr int WW = N == 64 ? 2 : 1;
+    const int tid = threadIdx.x;
+    const FillWork &w = W.f;
+    Halo H{0, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (W.halo[k] == kHaloRecompute) H.sides |= (uint32_t)k << (2 * H.n++);
+    for (int i = tid; i < 4 * N; i += kBlock) {
+        const int k = i / N, y = i - k * N;
+        if (W.halo[k] == kHaloRecompute) continue;
+        R v = 0;
+        const uint32_t *p = W.m.nb[k];
+        if (W.halo[k] == kHaloResident && p) {
+            if (WW == 2) v = (R)p[2 * y] | ((R)p[2 * y + 1] << (WW == 2 ? 32 : 0));
+            else v = (R)p[y];
+        }
+        S.nb[k][y] = v;
+    }
+    generate_rows<N, MAXN, false>(T, w.cx * N, w.cy * N, w.cz * N, H, S, nullptr);
+    __syncthreads();
+    write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
+    const uint32_t F = count_rows<N, MAXN>(S);
+    if (tid == 0) {
+        uint64_t prefix = 0;
+        if (t > 0) {
+            status_store(ctl.status + t, kFlagAgg | F);
+            int j = t - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                const uint64_t st = status_load(ctl.status + j);
+                const uint64_t flag = st & ~kValMask;
+                if (flag == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(ctl.error, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += st & kValMask;
+                if (flag == kFlagIncl || j == 0) break;
+                j--;
+            }
+        }
+        status_store(ctl.status + t, kFlagIncl | (prefix + F));
+        ctl.counts[t] = F;
+        ctl.bases[t] = prefix;
+        if (t == n - 1) *ctl.total = prefix + F;
+        S.base = prefix;
+    }
+    __syncthreads();
+    const uint64_t base = S.base;
+    if (base + F > out.capacity) {
+        if (tid == 0) atomicOr(ctl.overflow, 1u);
+    } else {
+        emit_rows<N, MAXN>(W.m, base, F, out, S);
+    }
+    __syncthreads();
+}
+
+// Register cap: three waves per SIMD (the noise and emit phases together
+// otherwise need ~185 VGPRs at MAXN = 32 and leave two).
+template <int MAXN>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8)))
+fused_kernel(const FusedWork *__restrict__ work, int32_t n,
+                                                       const TerrainDev *__restrict__ terrains,
+                                                       const GradLut *__restrict__ lut,
+                                                       const MeshTables *__restrict__ tables,
+                                                       FusedCtl ctl, MeshOut out) {
+    __shared__ FusedShared<MAXN> S;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
+    for (int i = tid; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = tid; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = tid; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
+    if (tid == 0) S.terrain = -1;
+    for (;;) {
+        if (tid == 0) S.item = (int)atomicAdd(ctl.ticket, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(S.item);  // uniform: descriptor loads go scalar
+        if (t >= n) break;
+        const FusedWork &W = work[t];
+        const int terrain = W.f.terrain;
+        if (terrain != S.terrain) {
+            for (int i = tid; i < 512; i += kBlock) S.perm[i] = terrains[terrain].perm[i];
+            __syncthreads();
+            if (tid == 0) S.terrain = terrain;
+        }
+        __syncthreads();
+        const TerrainDev &T = terrains[terrain];
+        switch (W.f.edge) {
+        case 16: fused_chunk<16, MAXN>(W, T, t, n, ctl, out, S); break;
+        case 32: if (MAXN >= 32) fused_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(W, T, t, n, ctl, out, S); break;
+        case 64: if (MAXN >= 64) fused_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(W, T, t, n, ctl, out, S); break;
+        default: break;
+        }
+    }
+}
+
+// Zeroes the control words of a fused run (a kernel rather than a memset so
+// the graph that replays the run holds kernel nodes only).
+__global__ void __launch_bounds__(kBlock) fused_reset_kernel(FusedCtl ctl, int32_t n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) ctl.status[i] = 0ull;
+    if (i == 0) {
+        *ctl.ticket = 0u;
+        *ctl.overflow = 0u;
+        *ctl.error = 0u;
+        *ctl.total = 0ull;
+    }
 }
 
 // --------------------------------------------------------------- frustum
@@ -753,6 +1003,25 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
         emit_kernel<32><<<n, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow);
     else
         emit_kernel<64><<<n, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow);
+    return hipGetLastError();
+}
+
+// Grid of the fused kernel: one workgroup per chunk.  Workgroups that stay
+// resident keep taking tickets; ones dispatched after the tickets run out
+// exit at once, so the grid size only bounds the parallelism.
+static int fused_grid(int n) { return n; }
+
+hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
+                        const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
+                        FusedCtl ctl, MeshOut out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    fused_reset_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(ctl, n);
+    if (max_edge <= 16)
+        fused_kernel<16><<<fused_grid(n), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out);
+    else if (max_edge <= 32)
+        fused_kernel<32><<<fused_grid(n), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out);
+    else
+        fused_kernel<64><<<fused_grid(n), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out);
     return hipGetLastError();
 }
 

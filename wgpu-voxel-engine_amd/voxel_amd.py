@@ -31,6 +31,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_CAPACITY, ERR_BUDGET, ERR_NOT_READY,
 TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY = range(4)
 FMT_INSTANCES, FMT_VERTICES = 1, 2
 FMT_INDICES = 4  # vx_batch_read_mesh selector for the index buffer
+BATCH_AUTO, BATCH_FUSED, BATCH_STAGED = 0, 1, 2
 AIR, GRASS, DIRT = 0, 1, 2
 
 INSTANCE_DTYPE = np.dtype([("m", "<f4", (4, 4)), ("tex", "<u4")])  # InstanceRaw, 68 B
@@ -145,6 +146,8 @@ _SIGS = {
     "vx_batch_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
     "vx_batch_destroy": (C.c_int, [P]),
     "vx_batch_run": (C.c_int, [P, P]),
+    "vx_batch_set_mode": (C.c_int, [P, C.c_int32]),
+    "vx_batch_get_mode": (C.c_int, [P, C.POINTER(C.c_int32)]),
     "vx_batch_run_per_chunk": (C.c_int, [P, P]),
     "vx_batch_wait": (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
     "vx_batch_chunk_faces": (C.c_int, [P, P, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -438,6 +441,15 @@ class Batch:
                 self.close()
         except Exception:
             pass
+
+    def set_mode(self, mode: int):
+        self.ctx.check(self.ctx.L.vx_batch_set_mode(self.h, mode), "vx_batch_set_mode")
+
+    @property
+    def mode(self) -> int:
+        m = C.c_int32()
+        self.ctx.check(self.ctx.L.vx_batch_get_mode(self.h, C.byref(m)), "vx_batch_get_mode")
+        return m.value
 
     def run(self):
         self.ctx.check(self.ctx.L.vx_batch_run(self.ctx.h, self.h), "vx_batch_run")
