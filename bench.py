@@ -92,6 +92,104 @@ def cpu_baseline(target_s: float = 12.0):
                       f"{reps} runs on {threads} threads ({cpu_model})"}
 
 
+def frame_leg(ctx, faces_per_chunk: float, frames: int = 20):
+    """Full frame (config 5): 65,536 resident 32^3 chunks (256 x 256, fBm seed
+    5) plus, per frame, 512 new chunks (the columns x = 256, 257 beside the
+    grid; their -x neighbours resident, +-z in the same batch) meshed and the
+    65,536 resident AABBs culled against a moving seeded camera (fov 70 deg,
+    near 0.1, far 1024) in one launch."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import test_oracle as T  # camera helpers only (look_at / perspective)
+
+    t5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64, lacunarity=2.0,
+                   gain=0.5, y_scale=1 / 128)
+    tid = ctx.terrain(t5)
+    side = 256
+    rc = np.array([(x, 0, z) for x in range(side) for z in range(side)], np.int32)
+    cap = int(len(rc) * faces_per_chunk * 1.15) + 4096
+    R = V.Batch(ctx, rc, EDGE, tid, V.grid_neighbours(side, side), face_capacity=cap)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    R.run()
+    resident_faces = R.wait()
+    gen_s = time.perf_counter() - t0
+    ext = [R.chunk((side - 1) * side + z) for z in range(side)]
+    nc = [(side + dx, 0, z) for dx in range(2) for z in range(side)]
+    nb = []
+    for dx in range(2):
+        for z in range(side):
+            i = dx * side + z
+            nb.append([(-2 - z) if dx == 0 else (i - side), (i + side) if dx == 0 else -1,
+                       i - 1 if z > 0 else -1, i + 1 if z < side - 1 else -1])
+    B = V.Batch(ctx, nc, EDGE, tid, nb, formats=V.FMT_VERTICES, ext=ext)
+    cbuf = V.DeviceBuffer.of(ctx, rc)
+    mbuf = V.DeviceBuffer(ctx, len(rc))
+    rng = np.random.default_rng(5)
+    eye = np.array([4096.0, 60.0, 4096.0])
+    heading = 0.0
+
+    def camera():
+        nonlocal eye, heading
+        heading += rng.normal(0, 0.05)
+        d = np.array([np.cos(heading), -0.2, np.sin(heading)])
+        eye = eye + 2.0 * d
+        m = T.colmajor(T.perspective(np.radians(70), 16 / 9, 0.1, 1024) @ T.look_at(eye, eye + d))
+        return V.Frustum(m)
+
+    B.set_cull(camera(), cbuf, len(rc), EDGE, mbuf)
+    B.run()
+    new_faces = B.wait()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        B.set_cull(camera(), cbuf, len(rc), EDGE, mbuf)
+        B.run()
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    assert B.wait() == new_faces
+    visible = int(mbuf.download(np.uint8, len(rc)).sum())
+    form = "fused" if B.mode == V.BATCH_FUSED else "staged"
+    B.close()
+    R.close()
+    for c in ext:
+        c.release()
+    cbuf.free()
+    mbuf.free()
+    return {"resident_chunks": len(rc), "resident_faces": resident_faces,
+            "resident_generation_ms": round(gen_s * 1e3, 2),
+            "resident_chunks_per_s": round(len(rc) / gen_s, 1),
+            "new_chunks_per_frame": len(nc), "new_faces_per_frame": new_faces,
+            "frame_ms": round(el * 1e3 / frames, 4), "frames": frames,
+            "visible_last_frame": visible, "form": form}
+
+
+def serving_leg(rates=(2000, 20000), duration=1.5):
+    """Streaming (config 4): camera random walk at 20 blocks/s (seed 3),
+    render radius 16 chunks of 32^3, Poisson arrivals, sizes 16/32/64
+    uniform, 8 GB HBM admission budget with LRU eviction."""
+    out = []
+    with V.Context(0, hbm_budget=8 << 30) as c:
+        t = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
+                      gain=0.5, y_scale=1 / 128)
+        tid = c.terrain(t)
+        for rate in rates:
+            r = V.simulate(c, seed=3, rate=rate, duration=duration, speed=20.0, radius=16,
+                           classes=((16, 1.0, tid), (32, 1.0, tid), (64, 1.0, tid)))
+            out.append({"rate_req_s": rate, "requests": r.requests, "completed": r.completed,
+                        "cache_hits": r.cache_hits, "generated": r.generated,
+                        "batches": r.batches, "evictions": r.evictions,
+                        "throughput_req_s": round(r.throughput_rps, 1),
+                        "generated_chunks_s": round(r.generated_per_s, 1),
+                        "p50_ms": round(r.p50_ms, 4), "p99_ms": round(r.p99_ms, 4),
+                        "max_ms": round(r.max_ms, 4),
+                        "miss_p50_ms": round(r.miss_p50_ms, 4),
+                        "miss_p99_ms": round(r.miss_p99_ms, 4),
+                        "peak_resident_GB": round(r.peak_resident_bytes / 2**30, 3)})
+            log(f"[bench] serving {rate} req/s: p50 {r.p50_ms:.3f} ms, p99 {r.p99_ms:.3f} ms "
+                f"(misses p99 {r.miss_p99_ms:.3f} ms), {r.generated} generated")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +198,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-chunk GPU path")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--no-frame", action="store_true", help="skip the full-frame leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the streaming leg")
+    ap.add_argument("--pmc-run", choices=["fused", "staged"], default=None,
+                    help="counter-collection run: 2 runs of the batch in one form, then a "
+                         "256 MiB plain copy kernel (the byte-counter reference); no timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -108,16 +211,31 @@ def main():
         sys.exit(2)
 
     ctx = V.Context(0)
+    if args.pmc_run:
+        t, coords, nb = workload(ctx)
+        b = V.Batch(ctx, coords, EDGE, ctx.terrain(t), nb, formats=V.FMT_VERTICES)
+        b.set_mode(V.BATCH_FUSED if args.pmc_run == "fused" else V.BATCH_STAGED)
+        b.run()
+        faces = b.wait()
+        b.run()
+        b.wait()
+        nbytes = 256 << 20
+        src, dst = V.DeviceBuffer(ctx, nbytes), V.DeviceBuffer(ctx, nbytes)
+        for _ in range(2):
+            ctx.check(ctx.L.vx_buffer_copy(ctx.h, dst.ptr, src.ptr, nbytes), "vx_buffer_copy")
+        ctx.synchronize()
+        print(json.dumps({"pmc_run": args.pmc_run, "faces": faces, "copy_bytes": nbytes}))
+        return
     t, coords, nb = workload(ctx)
     tid = ctx.terrain(t)
     n = len(coords)
     log(f"[bench] creating batch: {n} chunks of {EDGE}^3")
     b = V.Batch(ctx, coords, EDGE, tid, nb, formats=V.FMT_VERTICES)
-    b.set_mode(V.BATCH_AUTO)
-    mode = b.mode
     b.run()
     total_faces = b.wait()           # sizes the mesh arena (grows once if needed)
-    log(f"[bench] form: {'fused' if mode == V.BATCH_FUSED else 'staged'}; faces per step: {total_faces}")
+    tune = b.autotune(5)             # picks the form and launch shape for this batch
+    mode = b.mode
+    log(f"[bench] autotune: {tune}; faces per step: {total_faces}")
 
     def timed(steps, warmup):
         for _ in range(warmup):
@@ -144,20 +262,20 @@ def main():
     emit_bytes = n * (N3 // 8 + 4 * N2 // 8 + 4 + 8) + 104 * total_faces
     fused_bytes = fill_bytes + 104 * total_faces + n * 12       # one pass: writes only
 
-    # staged form for comparison (same outputs, four kernels)
-    staged = None
-    stage_ms = None
-    if b.mode == V.BATCH_FUSED:
-        b.set_mode(V.BATCH_STAGED)
-        b.run()
-        assert b.wait() == total_faces
-        se = timed(max(5, args.steps // 2), 1)
-        staged = {"value": round(n * max(5, args.steps // 2) / se, 1), "unit": "chunks/s",
-                  "ms_per_step": round(se * 1e3 / max(5, args.steps // 2), 4)}
-        stage_ms = b.profile(args.profile_reps)
-        b.set_mode(V.BATCH_AUTO)
-    else:
-        stage_ms = chosen_ms
+    # the other form, for comparison (same outputs)
+    other = None
+    reps2 = max(5, args.steps // 2)
+    b.set_mode(V.BATCH_STAGED if mode == V.BATCH_FUSED else V.BATCH_FUSED)
+    b.run()
+    assert b.wait() == total_faces
+    oe = timed(reps2, 1)
+    other_ms = b.profile(args.profile_reps)
+    other = {"form": "staged" if mode == V.BATCH_FUSED else "fused",
+             "value": round(n * reps2 / oe, 1), "unit": "chunks/s",
+             "ms_per_step": round(oe * 1e3 / reps2, 4)}
+    b.set_mode(V.BATCH_AUTO)
+    stage_ms = other_ms if mode == V.BATCH_FUSED else chosen_ms
+    fused_ms = chosen_ms if mode == V.BATCH_FUSED else other_ms
     names = ["fill", "count", "scan", "emit"]
     log("[bench] staged ms: " + ", ".join(f"{k}={v:.4f}" for k, v in zip(names, stage_ms[:4])))
     stages = {
@@ -171,26 +289,28 @@ def main():
         "algorithmic_bytes": {"fill": fill_bytes, "emit": emit_bytes, "fused": fused_bytes},
         "algorithmic_flops": fill_flops,
     }
+    k_ms = float(fused_ms[0])
+    f_tf = fill_flops / (k_ms * 1e-3) / 1e12
+    f_gbs = fused_bytes / (k_ms * 1e-3) / 1e9
+    stages["fused_kernel"] = {"ms": round(k_ms, 4), "TFLOP/s": round(f_tf, 2), "GB/s": round(f_gbs, 1)}
+    log(f"[bench] fused kernel {k_ms:.4f} ms: {f_tf:.1f} TFLOP/s, {f_gbs:.0f} GB/s")
     if mode == V.BATCH_FUSED:
-        k_ms = float(chosen_ms[0])
-        tf = fill_flops / (k_ms * 1e-3) / 1e12
-        gbs = fused_bytes / (k_ms * 1e-3) / 1e9
-        log(f"[bench] fused kernel {k_ms:.4f} ms: {tf:.1f} TFLOP/s, {gbs:.0f} GB/s")
         roof = {"kernel": "fused_kernel<32> (noise+fill+cull+scan+emit)", "bound": "valu",
-                "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "ms": round(k_ms, 4), "hbm_GB/s": round(gbs, 1),
-                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+                "achieved": round(f_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(f_tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "ms": round(k_ms, 4), "hbm_GB/s": round(f_gbs, 1),
+                "hbm_frac": round(f_gbs / HBM_PEAK_GBS, 4)}
     elif stage_ms[0] >= stage_ms[3]:
         ach = fill_flops / (stage_ms[0] * 1e-3) / 1e12
         roof = {"kernel": "fill_kernel<32> (noise+fill)", "bound": "valu", "achieved": round(ach, 2),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
-                "traffic": None}
+                "traffic": None, "ms": round(float(stage_ms[0]), 4)}
     else:
         ach = emit_bytes / (stage_ms[3] * 1e-3) / 1e9
         roof = {"kernel": "emit_kernel<32> (cull+compact+emit)", "bound": "hbm",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "ms": round(float(stage_ms[3]), 4)}
 
     plain = None
     if not args.no_plain:
@@ -207,6 +327,17 @@ def main():
                  "ms_per_step": round(pe * 1e3 / reps, 3),
                  "launches_per_step": 3 * n + 1}
         log(f"[bench] plain per-chunk path: {plain['value']:.0f} chunks/s")
+
+    frame = None
+    if not args.no_frame:
+        log("[bench] full frame: 65,536 resident + 512 new chunks per frame + cull...")
+        frame = frame_leg(ctx, total_faces / n)
+        log(f"[bench] frame: {frame['frame_ms']} ms ({frame['form']}), resident generation "
+            f"{frame['resident_chunks_per_s']:.0f} chunks/s")
+
+    serving = None
+    if not args.no_serve:
+        serving = serving_leg()
 
     cpu = None
     if not args.no_cpu:
@@ -235,9 +366,12 @@ def main():
         "roofline": roof,
         "stages": stages,
         "form": "fused" if mode == V.BATCH_FUSED else "staged",
+        "autotune": tune,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
-        "staged_gpu": staged,
+        "other_form_gpu": other,
+        "full_frame": frame,
+        "serving": serving,
     }
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)

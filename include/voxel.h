@@ -126,6 +126,17 @@ int vx_ctx_memory(vx_ctx *ctx, uint64_t *resident_bytes, uint64_t *budget_bytes)
 /* Last HIP error text recorded by the context (static storage). */
 const char *vx_ctx_last_error(vx_ctx *ctx);
 
+/* Device buffers for callers that keep inputs/outputs resident in HBM (e.g.
+ * chunk coordinates and cull masks of vx_frustum_mask_device). Allocated
+ * from the context heap and counted against its budget. */
+int vx_buffer_alloc(vx_ctx *ctx, uint64_t bytes, void **dev);
+int vx_buffer_free(vx_ctx *ctx, void *dev);
+int vx_buffer_upload(vx_ctx *ctx, void *dev, const void *host, uint64_t bytes);
+int vx_buffer_download(vx_ctx *ctx, void *host, const void *dev, uint64_t bytes);
+/* Device-to-device copy by a plain kernel (16-byte aligned, multiple of 16
+ * bytes); queued on the context stream. */
+int vx_buffer_copy(vx_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
 /* Registers a terrain generator; returns its id in *terrain_id. */
 int vx_terrain_register(vx_ctx *ctx, const vx_terrain *terrain, int32_t *terrain_id);
 
@@ -198,6 +209,20 @@ typedef struct vx_batch_req {
 
 int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n, uint32_t formats,
                     uint64_t face_capacity, vx_batch **out);
+/* As vx_batch_create, with resident neighbours: nb[k] = -2 - e names ext[e]
+ * (a chunk of the same edge, e.g. a view from vx_batch_chunk or a server
+ * lookup), retained by the batch. */
+int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n, vx_chunk *const *ext,
+                        int32_t n_ext, uint32_t formats, uint64_t face_capacity, vx_batch **out);
+/* Chunk i of the batch as a retained chunk handle (a view into the batch's
+ * storage, valid after the batch is destroyed). */
+int vx_batch_chunk(vx_batch *batch, int32_t i, vx_chunk **out);
+/* Frustum job run with every later run of the batch (in the fused form,
+ * inside the same launch): mask_dev[i] = contains(coords_dev[3i..3i+2]).
+ * Each call uploads new planes (a camera move needs no graph re-capture);
+ * n = 0 removes the job.  coords_dev / mask_dev are device pointers. */
+int vx_batch_set_cull(vx_batch *batch, const vx_frustum *f, const int32_t *coords_dev, int32_t n,
+                      int32_t edge, uint8_t *mask_dev);
 int vx_batch_destroy(vx_batch *batch);
 /* Execution form of a batch:
  *   VX_BATCH_AUTO    fused when eligible, else staged (default)
@@ -232,6 +257,23 @@ int vx_batch_device_buffers(vx_batch *batch, void **instances, void **vertices, 
  * fused form: ms[0] = fused kernel (with its control reset), ms[1..3] = 0;
  * ms[4] = whole run replayed from the graph. */
 int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
+/* Autotuner: times every candidate execution form and launch shape of this
+ * batch (staged; fused with one workgroup per chunk; fused with 2..4
+ * persistent workgroups per CU) over `reps` graph replays each, keeps the
+ * fastest for the batch, and caches the choice for batches of the same shape
+ * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
+ * VX_TUNE_CACHE set to a file path the cache persists across processes.
+ * Batches created later in VX_BATCH_AUTO use a cached choice. */
+typedef struct vx_tune_result {
+    int32_t mode;  /* chosen VX_BATCH_FUSED / VX_BATCH_STAGED */
+    int32_t grid;  /* chosen fused workgroups (0 = one per chunk) */
+    float best_ms;
+    int32_t n_candidates;
+    int32_t cand_mode[8];
+    int32_t cand_grid[8];
+    float cand_ms[8];
+} vx_tune_result;
+int vx_batch_autotune(vx_ctx *ctx, vx_batch *batch, int32_t reps, vx_tune_result *result);
 /* Plain path for comparison: the same work as one launch per chunk per
  * stage (no batching, no graph), chunk order preserved. */
 int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *batch);

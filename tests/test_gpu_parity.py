@@ -313,3 +313,55 @@ def test_frustum_mask_matches_oracle(ctx, oracle_mod):
         ref = oracle_mod.frustum_contains(f.planes, coords, 32)
         assert (got == ref).all()
         assert got.sum() > 0
+
+
+# ------------------------------------------------------------------ frame
+@pytest.mark.parametrize("mode", MODES)
+def test_frame_new_chunks_beside_resident_grid_with_cull(ctx, oracle_mod, mode):
+    """Config 5 shape at small scale: a resident 8x8 grid; a frame meshes 8
+    new chunks in the column x = 8 (their -x neighbours are resident chunks,
+    +-z neighbours are in the same batch) and culls the resident AABBs in
+    the same run."""
+    import test_oracle as T
+    say("frame", mode)
+    t = oterrain(oracle_mod, C5)
+    tid = ctx.terrain(C5)
+    rc = [(x, 0, z) for x in range(8) for z in range(8)]
+    R = V.Batch(ctx, rc, 32, tid, V.grid_neighbours(8, 8))
+    R.run()
+    R.wait()
+    ext = [R.chunk(7 * 8 + z) for z in range(8)]  # resident column x = 7
+    nc = [(8, 0, z) for z in range(8)]
+    nb = [[-2 - z, -1, z - 1 if z > 0 else -1, z + 1 if z < 7 else -1] for z in range(8)]
+    B = V.Batch(ctx, nc, 32, tid, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES, ext=ext)
+    B.set_mode(mode)
+    coords = np.array(rc, np.int32)
+    cbuf = V.DeviceBuffer.of(ctx, coords)
+    mbuf = V.DeviceBuffer(ctx, len(rc))
+    eye = np.array([-40.0, 30.0, 100.0])
+    for frame in range(2):  # the camera moves: new planes, same captured graph
+        m = T.colmajor(T.perspective(np.radians(70), 1.5, 0.1, 1024) @
+                       T.look_at(eye, eye + np.array([1.0, -0.3, -0.5 - frame])))
+        f = V.Frustum(m)
+        B.set_cull(f, cbuf, len(rc), 32, mbuf)
+        B.run()
+        total = B.wait()
+        mask = mbuf.download(np.uint8, len(rc))
+        assert (mask == oracle_mod.frustum_contains(f.planes, coords, 32)).all()
+        assert 0 < mask.sum() < len(rc)
+    ids = {c: oracle_mod.chunk_fill(t, *c, 32) for c in nc + [(7, 0, z) for z in range(8)]}
+    verts, insts = B.mesh(V.FMT_VERTICES), B.mesh(V.FMT_INSTANCES)
+    run = 0
+    for i, c in enumerate(nc):
+        adj = [ids[(7, 0, c[2])], None, ids.get((8, 0, c[2] - 1)), ids.get((8, 0, c[2] + 1))]
+        faces, inst, vert, idx = oracle_mod.mesh_buffers(ids[c], 32, (c[0] * 32, 0, c[2] * 32), adj)
+        first, cnt = B.chunk_faces(i)
+        assert first == run and cnt == len(faces)
+        assert (bits(verts[4 * first:4 * (first + cnt)]) == bits(vert)).all()
+        assert (bits(insts[first:first + cnt]) == bits(inst)).all()
+        run += cnt
+    assert total == run
+    B.close()
+    R.close()
+    cbuf.free()
+    mbuf.free()

@@ -371,6 +371,44 @@ extern "C" const char *vx_ctx_last_error(vx_ctx *ctx) {
     return ctx->last_error.c_str();
 }
 
+extern "C" int vx_buffer_alloc(vx_ctx *ctx, uint64_t bytes, void **dev) {
+    if (!ctx || !dev || bytes == 0) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return ctx->dev_alloc(bytes, dev);
+}
+
+extern "C" int vx_buffer_free(vx_ctx *ctx, void *dev) {
+    if (!ctx || !dev) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no queued work may still use it
+    ctx->dev_free(dev);
+    return VX_OK;
+}
+
+extern "C" int vx_buffer_upload(vx_ctx *ctx, void *dev, const void *host, uint64_t bytes) {
+    if (!ctx || !dev || (!host && bytes)) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    VX_HIP(ctx, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VX_OK;
+}
+
+extern "C" int vx_buffer_download(vx_ctx *ctx, void *host, const void *dev, uint64_t bytes) {
+    if (!ctx || !dev || (!host && bytes)) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    VX_HIP(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VX_OK;
+}
+
+extern "C" int vx_buffer_copy(vx_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    if (!ctx || !dst || !src || bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    VX_HIP(ctx, launch_copy(dst, src, bytes, ctx->stream));
+    return VX_OK;
+}
+
 extern "C" int vx_terrain_register(vx_ctx *ctx, const vx_terrain *t, int32_t *id) {
     if (!ctx || !t || !id) return VX_ERR_INVALID;
     if (t->kind < VX_TERRAIN_AIR || t->kind > VX_TERRAIN_DENSITY) return VX_ERR_INVALID;

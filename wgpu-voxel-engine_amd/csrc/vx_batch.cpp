@@ -16,7 +16,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <numeric>
+#include <tuple>
 
 using namespace vx;
 
@@ -111,18 +115,26 @@ hipError_t enqueue_emit(vx_batch *b) {
 }
 
 bool use_fused(const vx_batch *b) {
-    return b->fused_ok && b->mode != VX_BATCH_STAGED;
+    if (!b->fused_ok || b->mode == VX_BATCH_STAGED) return false;
+    if (b->mode == VX_BATCH_AUTO && b->tuned) return b->tuned_mode == VX_BATCH_FUSED;
+    return true;
+}
+
+int32_t fused_grid_of(const vx_batch *b) {
+    return b->mode == VX_BATCH_AUTO && b->tuned ? b->tuned_grid : b->grid;
 }
 
 hipError_t enqueue_fused(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
     return launch_fused(b->fused_dev, b->n, b->max_edge, ctx->terrains_dev, ctx->lut_dev,
-                        ctx->tables_dev, b->fctl_ptrs, b->out, ctx->stream);
+                        ctx->tables_dev, b->fctl_ptrs, b->out, b->cull, fused_grid_of(b),
+                        ctx->stream);
 }
 
 hipError_t enqueue_all(vx_batch *b) {
     if (use_fused(b)) return enqueue_fused(b);
-    hipError_t e = enqueue_fill(b);
+    hipError_t e = launch_cull(b->cull, b->ctx->stream);
+    if (e == hipSuccess) e = enqueue_fill(b);
     if (e == hipSuccess) e = enqueue_count(b);
     if (e == hipSuccess) e = enqueue_scan(b);
     if (e == hipSuccess) e = enqueue_emit(b);
@@ -157,7 +169,12 @@ void free_batch(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
     (void)hipStreamSynchronize(ctx->stream);
     drop_graph(b);
-    ctx->dev_free(b->store);
+    vx_shared_unref_locked(ctx, b->store_blk);
+    for (vx_chunk *c : b->ext)
+        if (c->refs.fetch_sub(1) == 1) vx_chunk_destroy_locked(c);
+    ctx->dev_free(b->planes_dev);
+    if (b->planes_pin) (void)hipHostFree(b->planes_pin);
+    if (b->planes_ev) (void)hipEventDestroy(b->planes_ev);
     ctx->dev_free(b->fill_dev);
     ctx->dev_free(b->mesh_dev);
     ctx->dev_free(b->order_dev);
@@ -182,6 +199,53 @@ int fetch_results(vx_batch *b) {
     return VX_OK;
 }
 
+// ---- autotuning -------------------------------------------------------
+// Shape of a batch for the tuning cache: chunk count bucket (power of two),
+// largest edge, uniform or ragged, fused form eligible, has neighbours.
+using TuneKey = std::tuple<int, int, int, int, int>;
+struct TuneChoice {
+    int32_t mode, grid;
+};
+std::map<TuneKey, TuneChoice> g_tune;
+std::mutex g_tune_mu;
+bool g_tune_loaded = false;
+
+TuneKey tune_key(const vx_batch *b) {
+    int bucket = 0;
+    while ((1 << bucket) < b->n) bucket++;
+    bool nb = false;
+    for (const vx_batch_req &r : b->reqs)
+        for (int k = 0; k < 4; k++) nb |= r.nb[k] != -1;
+    return TuneKey{bucket, b->max_edge, b->uniform ? 1 : 0, b->fused_ok ? 1 : 0, nb ? 1 : 0};
+}
+
+// The cache persists across processes when VX_TUNE_CACHE names a file:
+// one line per shape, "bucket max_edge uniform fused_ok has_nb mode grid".
+void tune_load() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *path = getenv("VX_TUNE_CACHE");
+    if (!path) return;
+    FILE *f = fopen(path, "r");
+    if (!f) return;
+    int a, c, d, e, h, m, g;
+    while (fscanf(f, "%d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g) == 7)
+        g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g};
+    fclose(f);
+}
+
+void tune_save() {
+    const char *path = getenv("VX_TUNE_CACHE");
+    if (!path) return;
+    FILE *f = fopen(path, "w");
+    if (!f) return;
+    for (auto &kv : g_tune)
+        fprintf(f, "%d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first),
+                kv.second.mode, kv.second.grid);
+    fclose(f);
+}
+
 }  // namespace
 
 void vx_batch_unref(vx_batch *b) {
@@ -190,7 +254,15 @@ void vx_batch_unref(vx_batch *b) {
 
 extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n, uint32_t formats,
                                uint64_t face_capacity, vx_batch **out) {
-    if (!ctx || !out || n <= 0 || !reqs) return VX_ERR_INVALID;
+    return vx_batch_create_ext(ctx, reqs, n, nullptr, 0, formats, face_capacity, out);
+}
+
+extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
+                                   vx_chunk *const *ext, int32_t n_ext, uint32_t formats,
+                                   uint64_t face_capacity, vx_batch **out) {
+    if (!ctx || !out || n <= 0 || !reqs || n_ext < 0 || (n_ext && !ext)) return VX_ERR_INVALID;
+    for (int e = 0; e < n_ext; e++)
+        if (!ext[e] || ext[e]->ctx != ctx) return VX_ERR_INVALID;
     if (formats == 0 || (formats & ~(kFmtInstances | kFmtVertices))) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     for (int i = 0; i < n; i++) {
@@ -198,8 +270,12 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
         if (r.edge != 16 && r.edge != 32 && r.edge != 64) return VX_ERR_INVALID;
         if (r.terrain_id < 0 || r.terrain_id >= (int)ctx->terrains.size()) return VX_ERR_INVALID;
         for (int k = 0; k < 4; k++) {
-            if (r.nb[k] < -1 || r.nb[k] >= n || r.nb[k] == i) return VX_ERR_INVALID;
-            if (r.nb[k] >= 0 && reqs[r.nb[k]].edge != r.edge) return VX_ERR_INVALID;
+            const int j = r.nb[k];
+            const bool in_batch = j >= 0 && j < n && j != i;
+            const bool external = j <= -2 && -2 - j < n_ext;
+            if (!(in_batch || j == -1 || external)) return VX_ERR_INVALID;
+            if (in_batch && reqs[j].edge != r.edge) return VX_ERR_INVALID;
+            if (external && ext[-2 - j]->edge != r.edge) return VX_ERR_INVALID;
         }
     }
     vx_batch *b = new (std::nothrow) vx_batch();
@@ -228,6 +304,12 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
         return s;
     };
     if ((st = ctx->dev_alloc(off, &b->store))) return fail(st);
+    b->store_blk = new vx::SharedBlock();
+    b->store_blk->ptr = b->store;
+    for (int e = 0; e < n_ext; e++) {
+        ext[e]->refs.fetch_add(1);
+        b->ext.push_back(ext[e]);
+    }
     if ((st = ctx->dev_alloc((size_t)n * sizeof(FillWork), (void **)&b->fill_dev))) return fail(st);
     if ((st = ctx->dev_alloc((size_t)n * sizeof(MeshWork), (void **)&b->mesh_dev))) return fail(st);
     const size_t ctl_bytes = up256((size_t)n * 4) + up256((size_t)n * 8) + 256;
@@ -268,10 +350,12 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
         m.ids = fw[i].ids;
         m.solid_type = kBlockGrass;  // generated chunks hold only grass
         static const int which[4] = {1, 0, 3, 2};  // facing plane of the neighbour
+        const size_t plane = (size_t)r.edge * words_per_row(r.edge);
         for (int k = 0; k < 4; k++) {
             const int j = r.nb[k];
-            m.nb[k] = j >= 0 ? fw[j].planes + (size_t)which[k] * r.edge * words_per_row(r.edge)
-                             : nullptr;
+            if (j >= 0) m.nb[k] = fw[j].planes + which[k] * plane;
+            else if (j <= -2) m.nb[k] = ext[-2 - j]->planes + which[k] * plane;
+            else m.nb[k] = nullptr;
         }
     }
     // fused form: every in-batch neighbour must be the adjacent chunk with the
@@ -296,7 +380,9 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
             fu[i].f = fw[i];
             fu[i].m = mw[i];
             for (int k = 0; k < 4; k++)
-                fu[i].halo[k] = reqs[i].nb[k] >= 0 ? kHaloRecompute : kHaloAbsent;
+                fu[i].halo[k] = reqs[i].nb[k] >= 0    ? kHaloRecompute
+                                : reqs[i].nb[k] <= -2 ? kHaloResident
+                                                      : kHaloAbsent;
         }
         if ((st = ctx->dev_alloc((size_t)n * sizeof(FusedWork), (void **)&b->fused_dev)))
             return fail(st);
@@ -330,6 +416,16 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
         for (int i = 0; i < n; i++) cap += 4ull * reqs[i].edge * reqs[i].edge;
     }
     if ((st = alloc_arena(b, cap))) return fail(st);
+    {
+        std::lock_guard<std::mutex> tg(g_tune_mu);
+        tune_load();
+        auto it = g_tune.find(tune_key(b));
+        if (it != g_tune.end() && (it->second.mode != VX_BATCH_FUSED || b->fused_ok)) {
+            b->tuned = true;
+            b->tuned_mode = it->second.mode;
+            b->tuned_grid = it->second.grid;
+        }
+    }
     *out = b;
     return VX_OK;
 }
@@ -483,11 +579,132 @@ extern "C" int vx_batch_read_mesh(vx_ctx *ctx, vx_batch *b, uint32_t format, voi
     return VX_OK;
 }
 
+extern "C" int vx_batch_chunk(vx_batch *b, int32_t i, vx_chunk **out) {
+    if (!b || !out || i < 0 || i >= b->n) return VX_ERR_INVALID;
+    vx_ctx *ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    const vx_batch_req &r = b->reqs[i];
+    const Layout L = layout_of(r.edge);
+    char *c = (char *)b->store + b->chunk_off[i];
+    vx_chunk *ch = new (std::nothrow) vx_chunk();
+    if (!ch) return VX_ERR_NOMEM;
+    ch->ctx = ctx;
+    ch->coord = r.coord;
+    ch->edge = r.edge;
+    ch->shared = b->store_blk;
+    b->store_blk->refs.fetch_add(1);
+    ch->ids = (uint8_t *)c;
+    ch->occ = (uint32_t *)(c + L.occ);
+    ch->planes = (uint32_t *)(c + L.planes);
+    ch->meta = (uint32_t *)(c + L.meta);
+    ch->solid_type = kBlockGrass;
+    *out = ch;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_set_cull(vx_batch *b, const vx_frustum *f, const int32_t *coords_dev,
+                                 int32_t n, int32_t edge, uint8_t *mask_dev) {
+    if (!b || n < 0 || (n && (!f || !coords_dev || !mask_dev || edge <= 0))) return VX_ERR_INVALID;
+    vx_ctx *ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n == 0) {
+        if (b->cull.n != 0) drop_graph(b);
+        b->cull = CullJob{};
+        return VX_OK;
+    }
+    if (!b->planes_dev) {
+        int st = ctx->dev_alloc(sizeof(f->planes), (void **)&b->planes_dev);
+        if (st) return st;
+        VX_HIP(ctx, hipHostMalloc((void **)&b->planes_pin, sizeof(f->planes), hipHostMallocDefault));
+        VX_HIP(ctx, hipEventCreateWithFlags(&b->planes_ev, hipEventDisableTiming));
+    } else {
+        VX_HIP(ctx, hipEventSynchronize(b->planes_ev));  // previous frame's copy has run
+    }
+    memcpy(b->planes_pin, f->planes, sizeof(f->planes));
+    VX_HIP(ctx, hipMemcpyAsync(b->planes_dev, b->planes_pin, sizeof(f->planes),
+                               hipMemcpyHostToDevice, ctx->stream));
+    VX_HIP(ctx, hipEventRecord(b->planes_ev, ctx->stream));
+    CullJob job{b->planes_dev, coords_dev, mask_dev, n, edge};
+    if (job.coords != b->cull.coords || job.mask != b->cull.mask || job.n != b->cull.n ||
+        job.edge != b->cull.edge)
+        drop_graph(b);  // kernel arguments change; planes alone are read from memory
+    b->cull = job;
+    return VX_OK;
+}
+
 extern "C" int vx_batch_device_buffers(vx_batch *b, void **inst, void **vert, void **idx) {
     if (!b) return VX_ERR_INVALID;
     if (inst) *inst = b->out.instances;
     if (vert) *vert = b->out.vertices;
     if (idx) *idx = b->out.indices;
+    return VX_OK;
+}
+
+extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune_result *res) {
+    if (!ctx || !b || b->ctx != ctx || reps <= 0) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    int cus = 1;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    std::vector<TuneChoice> cands;
+    cands.push_back({VX_BATCH_STAGED, 0});
+    if (b->fused_ok) {
+        cands.push_back({VX_BATCH_FUSED, 0});
+        for (int per = 2; per <= 4; per++)
+            if (cus * per < b->n) cands.push_back({VX_BATCH_FUSED, cus * per});
+    }
+    hipEvent_t e0, e1;
+    VX_HIP(ctx, hipEventCreate(&e0));
+    VX_HIP(ctx, hipEventCreate(&e1));
+    vx_tune_result r{};
+    float best = 1e30f;
+    TuneChoice bestc = cands[0];
+    int st = VX_OK;
+    for (size_t c = 0; c < cands.size() && !st; c++) {
+        drop_graph(b);
+        b->mode = cands[c].mode;
+        b->grid = cands[c].grid;
+        if ((st = capture_graph(b))) break;
+        hipError_t e = hipGraphLaunch(b->graph_exec, ctx->stream);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+        for (int i = 0; i < reps && e == hipSuccess; i++) e = hipGraphLaunch(b->graph_exec, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e != hipSuccess) {
+            ctx->set_error(e, "vx_batch_autotune");
+            st = VX_ERR_DEVICE;
+            break;
+        }
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        t /= reps;
+        if (c < 8) {
+            r.cand_mode[c] = cands[c].mode;
+            r.cand_grid[c] = cands[c].grid;
+            r.cand_ms[c] = t;
+            r.n_candidates = (int32_t)c + 1;
+        }
+        if (t < best) best = t, bestc = cands[c];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    drop_graph(b);
+    b->mode = VX_BATCH_AUTO;
+    b->grid = 0;
+    b->tuned = true;
+    b->tuned_mode = bestc.mode;
+    b->tuned_grid = bestc.grid;
+    b->ran = true;
+    b->results = false;
+    if (st) return st;
+    r.mode = bestc.mode;
+    r.grid = bestc.grid;
+    r.best_ms = best;
+    {
+        std::lock_guard<std::mutex> tg(g_tune_mu);
+        g_tune[tune_key(b)] = bestc;
+        tune_save();
+    }
+    if (res) *res = r;
     return VX_OK;
 }
 

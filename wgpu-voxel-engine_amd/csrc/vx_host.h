@@ -106,6 +106,12 @@ struct vx_batch {
     std::vector<vx_batch_req> reqs;
     std::vector<size_t> chunk_off;  // byte offset of chunk i in `store`
     void *store = nullptr;          // ids | occ | planes | meta per chunk
+    vx::SharedBlock *store_blk = nullptr;  // owns `store`; chunk views retain it
+    std::vector<vx_chunk *> ext;    // resident neighbours named by requests (retained)
+    vx::CullJob cull{};             // optional frustum job run with the batch
+    float *planes_dev = nullptr;    // device copy of the cull planes
+    float *planes_pin = nullptr;    // pinned staging for per-frame planes
+    hipEvent_t planes_ev = nullptr; // the staging copy has been consumed
     vx::FillWork *fill_dev = nullptr;
     vx::MeshWork *mesh_dev = nullptr;
     int32_t *order_dev = nullptr;   // longest-first schedule (null when uniform)
@@ -118,6 +124,9 @@ struct vx_batch {
     void *arena = nullptr;
     vx::MeshOut out{};
     int32_t mode = 0;               // VX_BATCH_* requested
+    int32_t grid = 0;               // fused workgroups (0 = one per chunk)
+    bool tuned = false;             // an autotuned choice applies in VX_BATCH_AUTO
+    int32_t tuned_mode = 0, tuned_grid = 0;
     bool fused_ok = false;          // fused form eligible
     vx::FusedWork *fused_dev = nullptr;
     void *fctl = nullptr;           // ticket | error | status[n]

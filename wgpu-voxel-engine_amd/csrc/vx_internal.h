@@ -116,6 +116,17 @@ struct FusedCtl {
     uint32_t *error;     // set when a look-back wait exceeded its bound
 };
 
+// Frustum job carried by a fused launch: planes in device memory (updated
+// per frame without re-capturing the graph), n chunk coordinates, edge,
+// one mask byte per chunk.  n = 0 disables it.
+struct CullJob {
+    const float *planes;
+    const int32_t *coords;
+    uint8_t *mask;
+    int32_t n;
+    int32_t edge;
+};
+
 // Six frustum planes [near far left right top bottom] x (a b c d), passed
 // to the cull kernel by value.
 struct Planes {
@@ -142,7 +153,10 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
                        const MeshTables *tables, uint32_t *overflow, hipStream_t s);
 hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
                         const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
-                        FusedCtl ctl, MeshOut out, hipStream_t s);
+                        FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
+                        hipStream_t s);
+hipError_t launch_cull(const CullJob &job, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n, int32_t edge,
                           uint8_t *mask, hipStream_t s);
 

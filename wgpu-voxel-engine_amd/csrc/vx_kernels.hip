@@ -818,33 +818,45 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
     const uint32_t F = count_rows<N, MAXN>(S);
-    if (tid == 0) {
+    if (tid < 64) {  // wave 0: publish, then look back 64 predecessors per round trip
+        const int lane = tid;
         uint64_t prefix = 0;
         if (t > 0) {
-            status_store(ctl.status + t, kFlagAgg | F);
-            int j = t - 1;
+            if (lane == 0) status_store(ctl.status + t, kFlagAgg | F);
+            int hi = t - 1;
             uint32_t spins = 0;
             for (;;) {
-                const uint64_t st = status_load(ctl.status + j);
+                const int j = hi - lane;
+                const uint64_t st = j >= 0 ? status_load(ctl.status + j) : kFlagIncl;
                 const uint64_t flag = st & ~kValMask;
-                if (flag == 0) {
+                if (__ballot(flag == 0)) {  // some predecessor has not published yet
                     if (++spins > kSpinLimit) {
-                        atomicOr(ctl.error, 1u);
+                        if (lane == 0) atomicOr(ctl.error, 1u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += st & kValMask;
-                if (flag == kFlagIncl || j == 0) break;
-                j--;
+                // Flags only grow (0 -> aggregate -> inclusive) and an inclusive
+                // prefix implies every earlier chunk has published, so the
+                // nearest inclusive entry ends the walk.
+                const uint64_t incl = __ballot(flag == kFlagIncl);
+                const int stop = incl ? __ffsll((unsigned long long)incl) - 1 : 63;
+                uint64_t v = lane <= stop ? (st & kValMask) : 0ull;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+                prefix += v;
+                if (incl) break;
+                hi -= 64;
             }
         }
-        status_store(ctl.status + t, kFlagIncl | (prefix + F));
-        ctl.counts[t] = F;
-        ctl.bases[t] = prefix;
-        if (t == n - 1) *ctl.total = prefix + F;
-        S.base = prefix;
+        if (lane == 0) {
+            status_store(ctl.status + t, kFlagIncl | (prefix + F));
+            ctl.counts[t] = F;
+            ctl.bases[t] = prefix;
+            if (t == n - 1) *ctl.total = prefix + F;
+            S.base = prefix;
+        }
     }
     __syncthreads();
     const uint64_t base = S.base;
@@ -856,6 +868,37 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
     __syncthreads();
 }
 
+// Frustum test of resident chunk AABBs folded into the fused launch: every
+// workgroup culls a grid-stride slice before it takes chunk tickets
+// (frustum_culling.rs:16-40; dist = ((a*x + b*y) + c*z) + d, NaN passes).
+__device__ __forceinline__ uint8_t aabb_visible(const float *pl, int cx, int cy, int cz, int edge) {
+    const float px = (float)(cx * edge), py = (float)(cy * edge), pz = (float)(cz * edge);
+    const float e = (float)edge;
+    const float xs[2] = {px, px + e}, ys[2] = {py, py + e}, zs[2] = {pz, pz + e};
+    uint32_t any = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const float x = xs[c & 1], y = ys[(c >> 2) & 1], z = zs[(c >> 1) & 1];
+        uint32_t inside = 1;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const float dist = pl[4 * k] * x + pl[4 * k + 1] * y + pl[4 * k + 2] * z + pl[4 * k + 3];
+            inside &= !(dist < 0.0f);
+        }
+        any |= inside;
+    }
+    return (uint8_t)any;
+}
+
+__device__ __forceinline__ void cull_slice(const CullJob &job) {
+    float pl[24];
+#pragma unroll
+    for (int i = 0; i < 24; i++) pl[i] = job.planes[i];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < job.n; i += gridDim.x * kBlock)
+        job.mask[i] = aabb_visible(pl, job.coords[3 * i], job.coords[3 * i + 1],
+                                   job.coords[3 * i + 2], job.edge);
+}
+
 // Register cap: three waves per SIMD (the noise and emit phases together
 // otherwise need ~185 VGPRs at MAXN = 32 and leave two).
 template <int MAXN>
@@ -864,9 +907,11 @@ fused_kernel(const FusedWork *__restrict__ work, int32_t n,
                                                        const TerrainDev *__restrict__ terrains,
                                                        const GradLut *__restrict__ lut,
                                                        const MeshTables *__restrict__ tables,
-                                                       FusedCtl ctl, MeshOut out) {
+                                                       FusedCtl ctl, MeshOut out,
+                                                       CullJob cull) {
     __shared__ FusedShared<MAXN> S;
     const int tid = threadIdx.x;
+    if (cull.n > 0) cull_slice(cull);
     for (int i = tid; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
     for (int i = tid; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
     for (int i = tid; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
@@ -1006,22 +1051,50 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
     return hipGetLastError();
 }
 
-// Grid of the fused kernel: one workgroup per chunk.  Workgroups that stay
-// resident keep taking tickets; ones dispatched after the tickets run out
-// exit at once, so the grid size only bounds the parallelism.
-static int fused_grid(int n) { return n; }
+// Grid of the fused kernel: by default one workgroup per chunk.  Workgroups
+// that stay resident keep taking tickets; ones dispatched after the tickets
+// run out exit at once, so the grid size only bounds the parallelism (the
+// autotuner also tries persistent grids of a few workgroups per CU).
+static int fused_grid(int n, int grid) { return grid > 0 && grid < n ? grid : n; }
 
 hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
                         const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
-                        FusedCtl ctl, MeshOut out, hipStream_t s) {
+                        FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
+                        hipStream_t s) {
     if (n <= 0) return hipSuccess;
     fused_reset_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(ctl, n);
     if (max_edge <= 16)
-        fused_kernel<16><<<fused_grid(n), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out);
+        fused_kernel<16><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out, cull);
     else if (max_edge <= 32)
-        fused_kernel<32><<<fused_grid(n), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out);
+        fused_kernel<32><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out, cull);
     else
-        fused_kernel<64><<<fused_grid(n), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out);
+        fused_kernel<64><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, terrains, lut, tables, ctl, out, cull);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(kBlock) cull_kernel(CullJob job) { cull_slice(job); }
+
+// Plain device copy, 16 bytes per lane per step (also the reference point
+// for calibrating HBM byte counters: it reads and writes exactly `n16` x 16 B).
+__global__ void __launch_bounds__(kBlock) copy_kernel(uint4 *__restrict__ dst,
+                                                      const uint4 *__restrict__ src, uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * kBlock)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s) {
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return hipSuccess;
+    const uint64_t blocks = (n16 + kBlock - 1) / kBlock;
+    copy_kernel<<<(unsigned)(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
+        (uint4 *)dst, (const uint4 *)src, n16);
+    return hipGetLastError();
+}
+
+hipError_t launch_cull(const CullJob &job, hipStream_t s) {
+    if (job.n <= 0) return hipSuccess;
+    cull_kernel<<<(job.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(job);
     return hipGetLastError();
 }
 

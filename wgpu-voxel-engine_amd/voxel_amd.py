@@ -105,6 +105,12 @@ class SimResult(C.Structure):
             "miss_p50_ms", "miss_p99_ms", "miss_max_ms")] + [("peak_resident_bytes", C.c_uint64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("grid", C.c_int32), ("best_ms", C.c_float),
+                ("n_candidates", C.c_int32), ("cand_mode", C.c_int32 * 8),
+                ("cand_grid", C.c_int32 * 8), ("cand_ms", C.c_float * 8)]
+
+
 class BatchReq(C.Structure):
     _fields_ = [("coord", Coord), ("edge", C.c_int32), ("terrain_id", C.c_int32),
                 ("nb", C.c_int32 * 4)]
@@ -123,6 +129,11 @@ _SIGS = {
     "vx_ctx_memory": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "vx_ctx_last_error": (C.c_char_p, [P]),
     "vx_terrain_register": (C.c_int, [P, C.POINTER(TerrainC), C.POINTER(C.c_int32)]),
+    "vx_buffer_alloc": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
+    "vx_buffer_free": (C.c_int, [P, P]),
+    "vx_buffer_upload": (C.c_int, [P, P, P, C.c_uint64]),
+    "vx_buffer_download": (C.c_int, [P, P, P, C.c_uint64]),
+    "vx_buffer_copy": (C.c_int, [P, P, P, C.c_uint64]),
     "vx_chunk_new": (C.c_int, [P, C.c_int32, Coord, C.c_int32, C.POINTER(P)]),
     "vx_chunk_from_ids": (C.c_int, [P, Coord, C.c_int32, P, C.POINTER(P)]),
     "vx_chunk_retain": (C.c_int, [P]),
@@ -144,6 +155,10 @@ _SIGS = {
     "vx_frustum_mask": (C.c_int, [P, C.POINTER(FrustumC), P, C.c_int32, C.c_int32, P]),
     "vx_frustum_mask_device": (C.c_int, [P, C.POINTER(FrustumC), P, C.c_int32, C.c_int32, P]),
     "vx_batch_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
+    "vx_batch_create_ext": (C.c_int, [P, P, C.c_int32, P, C.c_int32, C.c_uint32, C.c_uint64,
+                                      C.POINTER(P)]),
+    "vx_batch_chunk": (C.c_int, [P, C.c_int32, C.POINTER(P)]),
+    "vx_batch_set_cull": (C.c_int, [P, C.POINTER(FrustumC), P, C.c_int32, C.c_int32, P]),
     "vx_batch_destroy": (C.c_int, [P]),
     "vx_batch_run": (C.c_int, [P, P]),
     "vx_batch_set_mode": (C.c_int, [P, C.c_int32]),
@@ -155,6 +170,7 @@ _SIGS = {
     "vx_batch_read_mesh": (C.c_int, [P, P, C.c_uint32, P, C.c_uint64]),
     "vx_batch_device_buffers": (C.c_int, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
     "vx_batch_profile": (C.c_int, [P, P, C.c_int32, C.POINTER(C.c_float)]),
+    "vx_batch_autotune": (C.c_int, [P, P, C.c_int32, C.POINTER(TuneResult)]),
     "vx_server_create": (C.c_int, [P, C.POINTER(ServerDesc), C.POINTER(P)]),
     "vx_server_destroy": (C.c_int, [P]),
     "vx_server_submit": (C.c_int, [P, P, C.c_int32, P]),
@@ -267,6 +283,39 @@ class Context:
         self.check(self.L.vx_chunk_noise(self.h, self.terrain(t), Coord(*coord), edge, _p(out), n),
                    "vx_chunk_noise")
         return out
+
+
+class DeviceBuffer:
+    """A device allocation from the context heap."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx, self.nbytes, self.ptr = ctx, nbytes, P()
+        ctx.check(ctx.L.vx_buffer_alloc(ctx.h, nbytes, C.byref(self.ptr)), "vx_buffer_alloc")
+
+    @classmethod
+    def of(cls, ctx: Context, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        b = cls(ctx, arr.nbytes)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.L.vx_buffer_upload(self.ctx.h, self.ptr, _p(arr), arr.nbytes),
+                       "vx_buffer_upload")
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.L.vx_buffer_download(self.ctx.h, _p(out), self.ptr, out.nbytes),
+                       "vx_buffer_download")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.L.vx_buffer_free(self.ctx.h, self.ptr)
+            self.ptr = P()
 
 
 class Chunk:
@@ -409,7 +458,7 @@ class Batch:
     sequence (fill -> count -> scan -> emit)."""
 
     def __init__(self, ctx: Context, coords, edges, terrain_ids, neighbours=None,
-                 formats: int = FMT_VERTICES, face_capacity: int = 0):
+                 formats: int = FMT_VERTICES, face_capacity: int = 0, ext=None):
         coords = np.asarray(coords, np.int32).reshape(-1, 3)
         n = len(coords)
         edges = np.broadcast_to(np.asarray(edges, np.int32), (n,))
@@ -424,8 +473,11 @@ class Batch:
             for k in range(4):
                 reqs[i].nb[k] = int(nb[i, k])
         h = P()
-        ctx.check(ctx.L.vx_batch_create(ctx.h, C.cast(reqs, P), n, formats, face_capacity,
-                                        C.byref(h)), "vx_batch_create")
+        ext = list(ext or [])
+        ext_arr = (P * max(1, len(ext)))(*[c.h for c in ext])
+        ctx.check(ctx.L.vx_batch_create_ext(ctx.h, C.cast(reqs, P), n, C.cast(ext_arr, P), len(ext),
+                                            formats, face_capacity, C.byref(h)),
+                  "vx_batch_create_ext")
         self.ctx, self.h, self.n, self.formats = ctx, h, n, formats
         self.edges = np.array(edges)
         self.total = None
@@ -441,6 +493,17 @@ class Batch:
                 self.close()
         except Exception:
             pass
+
+    def chunk(self, i: int) -> "Chunk":
+        h = P()
+        self.ctx.check(self.ctx.L.vx_batch_chunk(self.h, i, C.byref(h)), "vx_batch_chunk")
+        return Chunk(self.ctx, h)
+
+    def set_cull(self, frustum, coords: "DeviceBuffer", n: int, edge: int, mask: "DeviceBuffer"):
+        self.ctx.check(self.ctx.L.vx_batch_set_cull(
+            self.h, C.byref(frustum.c) if frustum is not None else None,
+            coords.ptr if coords is not None else None, n, edge,
+            mask.ptr if mask is not None else None), "vx_batch_set_cull")
 
     def set_mode(self, mode: int):
         self.ctx.check(self.ctx.L.vx_batch_set_mode(self.h, mode), "vx_batch_set_mode")
@@ -488,6 +551,16 @@ class Batch:
         self.ctx.check(self.ctx.L.vx_batch_read_mesh(self.ctx.h, self.h, fmt, _p(out), out.nbytes),
                        "vx_batch_read_mesh")
         return out
+
+    def autotune(self, reps: int = 5) -> dict:
+        r = TuneResult()
+        self.ctx.check(self.ctx.L.vx_batch_autotune(self.ctx.h, self.h, reps, C.byref(r)),
+                       "vx_batch_autotune")
+        names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged"}
+        return {"mode": names.get(r.mode, r.mode), "grid": r.grid, "best_ms": round(r.best_ms, 4),
+                "candidates": [{"mode": names.get(r.cand_mode[i], r.cand_mode[i]),
+                                "grid": r.cand_grid[i], "ms": round(r.cand_ms[i], 4)}
+                               for i in range(r.n_candidates)]}
 
     def profile(self, reps: int = 10) -> np.ndarray:
         ms = (C.c_float * 5)()
