@@ -16,9 +16,12 @@
  *   vx_batch_*              many ChunkGenerator jobs at once: the body of
  *                           pool.execute(Chunk::new + create_mesh)
  *                                                      src/chunk_builder.rs:80-84
- *   vx_submit / vx_poll     pool.execute / flume try_recv
- *                                                      src/chunk_builder.rs:80-86,310-332
- *   vx_world_*              World + ChunkGenerator     src/world.rs:14-58,
+ *   vx_server_submit /      pool.execute / flume try_recv
+ *   vx_server_pump /                                   src/chunk_builder.rs:80-86,310-332
+ *   vx_server_poll
+ *   vx_server_lookup        world.chunks / world.meshes src/world.rs:14-17
+ *   vx_serve_simulate       the World + ChunkGenerator streaming loop
+ *                                                      src/world.rs:14-58,
  *                                                      src/chunk_builder.rs:45-352
  *
  * Conventions

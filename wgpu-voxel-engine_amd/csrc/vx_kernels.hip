@@ -785,11 +785,17 @@ constexpr uint64_t kFlagAgg = 1ull << 62, kFlagIncl = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 24;
 
+// A status word carries its flag and its value together and no workgroup
+// reads anything else another workgroup wrote during the launch, so the
+// look-back needs coherence of the word only: relaxed agent-scope atomics.
+// Release/acquire would order every earlier store too, which on this part
+// means writing back the issuing XCD's whole L2 per publish (the emit phase
+// keeps it full of vertex data) and serialises the prefix chain behind it.
 __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t status_load(uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int N, int MAXN>
