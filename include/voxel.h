@@ -67,7 +67,6 @@ typedef struct vx_ctx vx_ctx;
 typedef struct vx_chunk vx_chunk;
 typedef struct vx_mesh vx_mesh;
 typedef struct vx_batch vx_batch;
-typedef struct vx_world vx_world;
 
 typedef struct vx_coord {
     int32_t x, y, z;
@@ -239,6 +238,10 @@ int vx_batch_destroy(vx_batch *batch);
 #define VX_BATCH_FUSED 1
 #define VX_BATCH_STAGED 2
 int vx_batch_set_mode(vx_batch *batch, int32_t mode);
+/* Workgroups of the fused launch in VX_BATCH_FUSED (0 = one per chunk;
+ * fewer = persistent workgroups that keep taking chunk tickets).  The staged
+ * form ignores it. */
+int vx_batch_set_grid(vx_batch *batch, int32_t grid);
 /* The form the next run uses (VX_BATCH_FUSED or VX_BATCH_STAGED). */
 int vx_batch_get_mode(vx_batch *batch, int32_t *mode);
 /* Queues fill -> count -> scan -> emit on the context stream (replayed from a
@@ -267,14 +270,15 @@ int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
  * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
  * VX_TUNE_CACHE set to a file path the cache persists across processes.
  * Batches created later in VX_BATCH_AUTO use a cached choice. */
+#define VX_TUNE_MAX_CANDIDATES 8
 typedef struct vx_tune_result {
     int32_t mode;  /* chosen VX_BATCH_FUSED / VX_BATCH_STAGED */
-    int32_t grid;  /* chosen fused workgroups (0 = one per chunk) */
+    int32_t grid;  /* chosen launch shape (vx_batch_set_grid) */
     float best_ms;
     int32_t n_candidates;
-    int32_t cand_mode[8];
-    int32_t cand_grid[8];
-    float cand_ms[8];
+    int32_t cand_mode[VX_TUNE_MAX_CANDIDATES];
+    int32_t cand_grid[VX_TUNE_MAX_CANDIDATES];
+    float cand_ms[VX_TUNE_MAX_CANDIDATES];
 } vx_tune_result;
 int vx_batch_autotune(vx_ctx *ctx, vx_batch *batch, int32_t reps, vx_tune_result *result);
 /* Plain path for comparison: the same work as one launch per chunk per

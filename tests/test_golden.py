@@ -77,14 +77,15 @@ def test_oracle_frustum_matches_golden(oracle_mod):
 
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [V.BATCH_FUSED, V.BATCH_STAGED])
+@pytest.mark.parametrize("mode,grid", [(V.BATCH_FUSED, 0), (V.BATCH_FUSED, 2), (V.BATCH_STAGED, 0)])
 @pytest.mark.parametrize("name", sorted(GOLDEN["batches"]))
-def test_library_batches_match_golden(ctx, name, mode):
+def test_library_batches_match_golden(ctx, name, mode, grid):
     g = GOLDEN["batches"][name]
     tid = ctx.terrain(lib_terrain(g["terrain"]))
     b = V.Batch(ctx, g["coords"], g["edges"], tid, g["neighbours"],
                 formats=V.FMT_VERTICES | V.FMT_INSTANCES)
     b.set_mode(mode)
+    b.set_grid(grid)
     b.run()
     assert b.wait() == sum(g["faces"])
     assert [b.chunk_faces(i)[1] for i in range(len(g["coords"]))] == g["faces"]

@@ -157,11 +157,18 @@ def test_generated_chunk_mesh_with_generated_neighbours(ctx, oracle_mod):
 
 
 # ------------------------------------------------------------------ batches
+def apply_mode(b, mode):
+    """mode: a VX_BATCH_* value or (mode, grid), grid = fused workgroups."""
+    m, g = mode if isinstance(mode, tuple) else (mode, 0)
+    b.set_mode(m)
+    b.set_grid(g)
+
+
 def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES,
                     face_capacity=0, per_chunk=False, mode=V.BATCH_AUTO):
     tids = [ctx.terrain(t) for t in terrains]
     b = V.Batch(ctx, coords, edges, tids, nb, formats=formats, face_capacity=face_capacity)
-    b.set_mode(mode)
+    apply_mode(b, mode)
     if per_chunk:
         b.run_per_chunk()
     else:
@@ -189,7 +196,11 @@ def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES 
     return b
 
 
-MODES = [V.BATCH_FUSED, V.BATCH_STAGED]
+# fused with one workgroup per chunk; fused with 3 persistent workgroups that
+# take every chunk ticket in turn; staged
+MODES = [pytest.param((V.BATCH_FUSED, 0), id="fused"),
+         pytest.param((V.BATCH_FUSED, 3), id="fused_persistent3"),
+         pytest.param((V.BATCH_STAGED, 0), id="staged")]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -202,7 +213,7 @@ def test_batch_config1_grid(ctx, oracle_mod, mode):
 
 
 @pytest.mark.parametrize("mode,per_chunk", [(V.BATCH_FUSED, False), (V.BATCH_STAGED, False),
-                                            (V.BATCH_STAGED, True)])
+                                            (V.BATCH_STAGED, True), ((V.BATCH_FUSED, 1), False)])
 def test_batch_config2_subgrid(ctx, oracle_mod, mode, per_chunk):
     """Config 2 terrain on an 8x8 sub-grid of 32^3 chunks."""
     say("batch C2 8x8", mode, per_chunk)
@@ -269,7 +280,7 @@ def test_batch_replay_is_stable(ctx, oracle_mod, mode):
     """hipGraph replays give identical results, with eager work between."""
     coords = [(x, 0, z) for x in range(4) for z in range(4)]
     b = V.Batch(ctx, coords, 32, ctx.terrain(C2), V.grid_neighbours(4, 4))
-    b.set_mode(mode)
+    apply_mode(b, mode)
     b.run()
     t0 = b.wait()
     v0 = b.mesh(V.FMT_VERTICES).copy()
@@ -334,7 +345,7 @@ def test_frame_new_chunks_beside_resident_grid_with_cull(ctx, oracle_mod, mode):
     nc = [(8, 0, z) for z in range(8)]
     nb = [[-2 - z, -1, z - 1 if z > 0 else -1, z + 1 if z < 7 else -1] for z in range(8)]
     B = V.Batch(ctx, nc, 32, tid, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES, ext=ext)
-    B.set_mode(mode)
+    apply_mode(B, mode)
     coords = np.array(rc, np.int32)
     cbuf = V.DeviceBuffer.of(ctx, coords)
     mbuf = V.DeviceBuffer(ctx, len(rc))

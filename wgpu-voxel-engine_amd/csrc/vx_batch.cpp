@@ -107,10 +107,11 @@ hipError_t enqueue_count(vx_batch *b) {
     return launch_count(b->mesh_dev, b->n, b->max_edge, b->counts, b->ctx->stream);
 }
 hipError_t enqueue_scan(vx_batch *b) {
-    return launch_scan(b->counts, b->n, b->bases, b->total, b->overflow, b->ctx->stream);
+    return launch_scan(b->counts, b->n, b->bases, b->total, b->overflow, b->emit_order,
+                       b->ctx->stream);
 }
 hipError_t enqueue_emit(vx_batch *b) {
-    return launch_emit(b->mesh_dev, b->order_dev, b->n, b->max_edge, b->bases, b->counts, b->out,
+    return launch_emit(b->mesh_dev, b->emit_order, b->n, b->max_edge, b->bases, b->counts, b->out,
                        b->ctx->tables_dev, b->overflow, b->ctx->stream);
 }
 
@@ -312,12 +313,13 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
     }
     if ((st = ctx->dev_alloc((size_t)n * sizeof(FillWork), (void **)&b->fill_dev))) return fail(st);
     if ((st = ctx->dev_alloc((size_t)n * sizeof(MeshWork), (void **)&b->mesh_dev))) return fail(st);
-    const size_t ctl_bytes = up256((size_t)n * 4) + up256((size_t)n * 8) + 256;
+    const size_t ctl_bytes = up256((size_t)n * 4) + up256((size_t)n * 8) + 256 + up256((size_t)n * 4);
     if ((st = ctx->dev_alloc(ctl_bytes, &b->ctl))) return fail(st);
     b->counts = (uint32_t *)b->ctl;
     b->bases = (uint64_t *)((char *)b->ctl + up256((size_t)n * 4));
     b->total = (uint64_t *)((char *)b->bases + up256((size_t)n * 8));
     b->overflow = (uint32_t *)(b->total + 1);
+    b->emit_order = (int32_t *)((char *)b->total + 256);
 
     std::vector<FillWork> fw(n);
     std::vector<MeshWork> mw(n);
@@ -446,6 +448,17 @@ extern "C" int vx_batch_set_mode(vx_batch *b, int32_t mode) {
         (void)hipStreamSynchronize(b->ctx->stream);
         drop_graph(b);
         b->mode = mode;
+    }
+    return VX_OK;
+}
+
+extern "C" int vx_batch_set_grid(vx_batch *b, int32_t grid) {
+    if (!b || grid < 0) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    if (grid != b->grid) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        drop_graph(b);
+        b->grid = grid;
     }
     return VX_OK;
 }
@@ -677,7 +690,7 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
         float t = 0;
         (void)hipEventElapsedTime(&t, e0, e1);
         t /= reps;
-        if (c < 8) {
+        if (c < VX_TUNE_MAX_CANDIDATES) {
             r.cand_mode[c] = cands[c].mode;
             r.cand_grid[c] = cands[c].grid;
             r.cand_ms[c] = t;

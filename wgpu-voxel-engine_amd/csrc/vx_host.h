@@ -115,7 +115,8 @@ struct vx_batch {
     vx::FillWork *fill_dev = nullptr;
     vx::MeshWork *mesh_dev = nullptr;
     int32_t *order_dev = nullptr;   // longest-first schedule (null when uniform)
-    void *ctl = nullptr;            // counts[n] | bases[n] | total | overflow
+    void *ctl = nullptr;            // counts[n] | bases[n] | total | overflow | emit_order[n]
+    int32_t *emit_order = nullptr;  // emission order written by the scan (largest first)
     uint32_t *counts = nullptr;
     uint64_t *bases = nullptr;
     uint64_t *total = nullptr;

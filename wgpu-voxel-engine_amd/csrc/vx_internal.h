@@ -141,7 +141,7 @@ hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, in
 hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint32_t *counts,
                         hipStream_t s);
 hipError_t launch_scan(const uint32_t *counts, int32_t n, uint64_t *bases, uint64_t *total,
-                       uint32_t *overflow, hipStream_t s);
+                       uint32_t *overflow, int32_t *order_out, hipStream_t s);
 hipError_t launch_fill_one(const FillWork &w, const TerrainDev *terrains, const GradLut *lut,
                            hipStream_t s);
 hipError_t launch_pack_one(const FillWork &w, hipStream_t s);

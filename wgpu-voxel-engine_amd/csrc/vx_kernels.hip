@@ -160,19 +160,25 @@ __device__ __forceinline__ void store_lines(SH &S, const Halo &H, uint64_t b, in
 // per octave, the y terms come from S.ytab, the 8 corner gradients are
 // rehashed only when the y lattice cell changes.  Voxel is solid iff
 // fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0 (oracle: vxo_chunk_fill).
-template <int N, int MAXN, bool DBG, class SH>
+// Work items are (line group, y segment of SEG rows) pairs dealt round-robin
+// to the waves, so that the halo lines do not leave some waves a whole extra
+// line group (N = 32 with four halo lines: 36 items of 16 rows, 9 per wave).
+template <int N, int MAXN, bool DBG, int SEG, class SH>
 __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
                               float *noise_out) {
     constexpr int L = 64 / N;
-    constexpr int SEG = N < 32 ? N : 32;
+    constexpr int NSEG = N / SEG;
+    static_assert(N % SEG == 0, "segments tile the column");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int G = (N + H.n + L - 1) / L;
     const int octaves = T.octaves;
-    for (int g = wave; g < G; g += kBlock / 64) {
+    for (int item = wave; item < G * NSEG; item += kBlock / 64) {
+        const int g = item / NSEG;
+        const int y0 = (item - g * NSEG) * SEG;
         const int li = g * L + lane / N;
         int wxi, wzi;
         line_column<N>(li, lane, H, ox, oz, wxi, wzi);
-        for (int y0 = 0; y0 < N; y0 += SEG) {
+        {
             float sum[SEG];
 #pragma unroll
             for (int i = 0; i < SEG; i++) sum[i] = 0.0f;
@@ -282,14 +288,14 @@ __device__ __forceinline__ void build_ytab(const TerrainDev &T, int oy, SH &S) {
 }
 
 // Fills S.occ (and the recomputed halo planes) for terrain T.
-template <int N, int MAXN, bool DBG, class SH>
+template <int N, int MAXN, bool DBG, int SEG, class SH>
 __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
                               float *noise_out) {
     using R = typename RowType<MAXN>::T;
     if (T.kind == kDensity) {
         build_ytab<N>(T, oy, S);
         __syncthreads();
-        density_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
+        density_lines<N, MAXN, DBG, SEG>(T, ox, oy, oz, H, S, noise_out);
     } else if (T.kind == kHeightmap) {
         heightmap_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
     } else {
@@ -363,7 +369,7 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
 template <int N, int MAXN, bool DBG>
 __device__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
     const Halo H{0, 0u};
-    generate_rows<N, MAXN, DBG>(T, w.cx * N, w.cy * N, w.cz * N, H, S, w.noise_out);
+    generate_rows<N, MAXN, DBG, (N < 32 ? N : 32)>(T, w.cx * N, w.cy * N, w.cz * N, H, S, w.noise_out);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
 }
@@ -455,6 +461,12 @@ __global__ void __launch_bounds__(kBlock) pack_one_kernel(FillWork w) {
 }
 
 // --------------------------------------------------------------- mesh
+// Per-wave staging of 64 face records (80 B of vertices or 68 B of
+// InstanceRaw each) so that global stores are lane-contiguous: a record per
+// lane written as 16-B pieces at an 80-B stride reaches about half the
+// HBM write rate of contiguous 16-B lanes.
+constexpr int kStageWords = 64 * 20;
+
 template <int MAXN> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
@@ -463,7 +475,15 @@ template <int MAXN> struct MeshShared {
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
     uint32_t red[kBlock / 64];
+    float4 stage[kBlock / 64][kStageWords / 4];
 };
+
+// Orders the LDS accesses of one wave: staging written by some lanes is read
+// by others of the same wave (LDS executes one wave's operations in order).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
 template <int N, int MAXN, class SH>
 __device__ __forceinline__ void load_rows(const MeshWork &w, SH &S) {
@@ -523,8 +543,15 @@ __device__ uint32_t count_rows(SH &S) {
     return block_sum(c, S.red);
 }
 
+// LDS of the count kernel: rows and neighbour planes only.
+template <int MAXN> struct CountShared {
+    typename RowType<MAXN>::T occ[MAXN * MAXN];
+    typename RowType<MAXN>::T nb[4][MAXN];
+    uint32_t red[kBlock / 64];
+};
+
 template <int N, int MAXN>
-__device__ void count_chunk(const MeshWork &w, MeshShared<MAXN> &S, uint32_t *count_out) {
+__device__ void count_chunk(const MeshWork &w, CountShared<MAXN> &S, uint32_t *count_out) {
     load_rows<N, MAXN>(w, S);
     __syncthreads();
     const uint32_t total = count_rows<N, MAXN>(S);
@@ -532,7 +559,7 @@ __device__ void count_chunk(const MeshWork &w, MeshShared<MAXN> &S, uint32_t *co
 }
 
 template <int MAXN>
-__device__ __forceinline__ void count_item(const MeshWork &w, MeshShared<MAXN> &S, uint32_t *out) {
+__device__ __forceinline__ void count_item(const MeshWork &w, CountShared<MAXN> &S, uint32_t *out) {
     switch (w.edge) {
     case 16: count_chunk<16, MAXN>(w, S, out); break;
     case 32: if (MAXN >= 32) count_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, S, out); break;
@@ -544,23 +571,35 @@ __device__ __forceinline__ void count_item(const MeshWork &w, MeshShared<MAXN> &
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) count_kernel(const MeshWork *__restrict__ work,
                                                        uint32_t *__restrict__ counts) {
-    __shared__ MeshShared<MAXN> S;
+    __shared__ CountShared<MAXN> S;
     count_item<MAXN>(work[blockIdx.x], S, counts + blockIdx.x);
 }
 
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) count_one_kernel(MeshWork w, uint32_t *__restrict__ count) {
-    __shared__ MeshShared<MAXN> S;
+    __shared__ CountShared<MAXN> S;
     count_item<MAXN>(w, S, count);
 }
 
+// Emission order: chunks by descending face count (buckets of 16 faces,
+// counts >= 16368 share the first bucket), so the emit grid starts with the
+// largest chunks and ends with small ones instead of ending on a long one.
+// Order inside a bucket is arbitrary; outputs depend only on the bases.
+constexpr int kOrderBuckets = 1024;
+__device__ __forceinline__ int order_bucket(uint32_t c) {
+    return kOrderBuckets - 1 - (int)min(c >> 4, (uint32_t)(kOrderBuckets - 1));
+}
+
 // Device-wide exclusive scan of per-chunk face counts (one workgroup of
-// 1024 threads, each owning a contiguous run of chunks).
+// 1024 threads, each owning a contiguous run of chunks); with order_out it
+// also writes the emission order (a counting sort by face count).
 __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__ counts, int32_t n,
                                                     uint64_t *__restrict__ bases,
                                                     uint64_t *__restrict__ total,
-                                                    uint32_t *__restrict__ overflow) {
+                                                    uint32_t *__restrict__ overflow,
+                                                    int32_t *__restrict__ order_out) {
     __shared__ uint64_t s_wave[16];
+    __shared__ uint32_t s_hist[kOrderBuckets];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = (n + 1023) / 1024;
     const int b = tid * per, e = min(n, b + per);
@@ -588,6 +627,27 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
         *total = sum;
         if (overflow) *overflow = 0u;  // the emit stage that follows sets it
     }
+    if (!order_out) return;
+    static_assert(kOrderBuckets == 1024, "one bucket per thread");
+    s_hist[tid] = 0u;
+    __syncthreads();
+    for (int i = b; i < e; i++) atomicAdd(&s_hist[order_bucket(counts[i])], 1u);
+    __syncthreads();
+    const uint32_t cnt = s_hist[tid];
+    uint32_t cincl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(cincl, d, 64);
+        if (lane >= d) cincl += t;
+    }
+    __shared__ uint32_t s_w32[16];
+    if (lane == 63) s_w32[wave] = cincl;
+    __syncthreads();
+    uint32_t cbase = 0;
+    for (int i = 0; i < wave; i++) cbase += s_w32[i];
+    s_hist[tid] = cbase + cincl - cnt;
+    __syncthreads();
+    for (int i = b; i < e; i++) order_out[atomicAdd(&s_hist[order_bucket(counts[i])], 1u)] = i;
 }
 
 // Emits one chunk's faces.  Each thread owns RPT consecutive rows; a block
@@ -650,43 +710,73 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             }
         }
         __syncthreads();
-        for (uint32_t q = tid; q < t1 - t0; q += kBlock) {
-            const uint32_t dsc = S.desc[q];
+        // Each wave takes 64 consecutive faces of the tile: lane i builds face
+        // q0 + i in registers, stages it in the wave's LDS slot, then the wave
+        // writes the 64 records as contiguous 16-B (vertices) or 4-B
+        // (instances) lanes.
+        const int lane = tid & 63, wave = tid >> 6;
+        const uint32_t nt = t1 - t0;
+        float4 *st4 = S.stage[wave];
+        uint32_t *stw = reinterpret_cast<uint32_t *>(S.stage[wave]);
+        for (uint32_t q0 = 64u * wave; q0 < nt; q0 += kBlock) {
+            const uint32_t q = q0 + lane;
+            const bool valid = q < nt;
+            const uint32_t nv = min(64u, nt - q0);
+            const uint32_t dsc = valid ? S.desc[q] : 0u;
             const int r = dsc & 0xFFF, x = (dsc >> 12) & 0x3F, d = (dsc >> 18) & 7;
             const int y = r / N, z = r - y * N;
             const int wx = w.ox + x, wy = w.oy + y, wz = w.oz + z;
             const float t[3] = {(float)wx, (float)wy, (float)wz};
             const int sg[3] = {wx < 0, wy < 0, wz < 0};
-            uint32_t blk = w.solid_type;
-            if (blk == kBlockMixed) blk = w.ids[(size_t)x + (size_t)N * z + (size_t)N * N * y];
-            const uint32_t tex = S.tex[blk][d];
-            const uint64_t face = base + t0 + q;
+            const uint64_t face0 = base + t0 + q0;
             if (want_v) {
-                float v[20];
+                if (valid) {
+                    float v[20];
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                    for (int c = 0; c < 4; c++) {
 #pragma unroll
-                    for (int k = 0; k < 3; k++) v[c * 5 + k] = S.P[d][sg[k]][c][k] + t[k];
-                    v[c * 5 + 3] = (c & 1) ? 1.0f : 0.0f;
-                    v[c * 5 + 4] = (c & 2) ? 1.0f : 0.0f;
+                        for (int k = 0; k < 3; k++) v[c * 5 + k] = S.P[d][sg[k]][c][k] + t[k];
+                        v[c * 5 + 3] = (c & 1) ? 1.0f : 0.0f;
+                        v[c * 5 + 4] = (c & 2) ? 1.0f : 0.0f;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 5; k++)
+                        st4[lane * 5 + k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
                 }
-                float4 *dst = reinterpret_cast<float4 *>(out.vertices + face * 20);
+                wave_sync();
+                float4 *dst = reinterpret_cast<float4 *>(out.vertices + face0 * 20);
 #pragma unroll
-                for (int k = 0; k < 5; k++)
-                    dst[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+                for (int k = 0; k < 5; k++) {
+                    const uint32_t i = 64u * k + lane;
+                    if (i < 5u * nv) dst[i] = st4[i];
+                }
+                wave_sync();
             }
             if (want_i) {
-                uint32_t *dst = out.instances + face * 17;
+                if (valid) {
+                    uint32_t blk = w.solid_type;
+                    if (blk == kBlockMixed)
+                        blk = w.ids[(size_t)x + (size_t)N * z + (size_t)N * N * y];
+                    uint32_t *rec = stw + lane * 17;
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
+                    for (int c = 0; c < 3; c++) {
 #pragma unroll
-                    for (int k = 0; k < 3; k++) dst[c * 4 + k] = __float_as_uint(S.Mb[d][sg[k]][k][c]);
-                    dst[c * 4 + 3] = 0u;  // M(3,c) = +0
+                        for (int k = 0; k < 3; k++) rec[c * 4 + k] = __float_as_uint(S.Mb[d][sg[k]][k][c]);
+                        rec[c * 4 + 3] = 0u;  // M(3,c) = +0
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; k++) rec[12 + k] = __float_as_uint(t[k]);
+                    rec[15] = 0x3F800000u;  // 1.0f
+                    rec[16] = S.tex[blk][d];
                 }
+                wave_sync();
+                uint32_t *dst = out.instances + face0 * 17;
 #pragma unroll
-                for (int k = 0; k < 3; k++) dst[12 + k] = __float_as_uint(t[k]);
-                dst[15] = 0x3F800000u;  // 1.0f
-                dst[16] = tex;
+                for (int k = 0; k < 17; k++) {
+                    const uint32_t i = 64u * k + lane;
+                    if (i < 17u * nv) dst[i] = stw[i];
+                }
+                wave_sync();
             }
         }
         __syncthreads();
@@ -776,6 +866,7 @@ template <int MAXN> struct FusedShared {
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
     uint32_t red[kBlock / 64];
+    float4 stage[kBlock / 64][kStageWords / 4];
     int item;
     int terrain;
     uint64_t base;
@@ -820,7 +911,9 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
         }
         S.nb[k][y] = v;
     }
-    generate_rows<N, MAXN, false>(T, w.cx * N, w.cy * N, w.cz * N, H, S, nullptr);
+    // Halo lines make N + 4 lines: shorter y segments keep the waves even.
+    generate_rows<N, MAXN, false, (N == 64 ? 32 : N / 2)>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
+                                                          nullptr);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
     const uint32_t F = count_rows<N, MAXN>(S);
@@ -1008,9 +1101,9 @@ hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint3
 }
 
 hipError_t launch_scan(const uint32_t *counts, int32_t n, uint64_t *bases, uint64_t *total,
-                       uint32_t *overflow, hipStream_t s) {
+                       uint32_t *overflow, int32_t *order_out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    scan_kernel<<<1, 1024, 0, s>>>(counts, n, bases, total, overflow);
+    scan_kernel<<<1, 1024, 0, s>>>(counts, n, bases, total, overflow, order_out);
     return hipGetLastError();
 }
 

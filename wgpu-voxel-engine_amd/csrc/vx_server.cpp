@@ -369,7 +369,11 @@ int admit(vx_server *S) {
     if (e == hipSuccess)
         e = launch_fill(F->fill_dev, F->order_dev, n, F->max_edge, ctx->terrains_dev, ctx->lut_dev, s);
     if (e == hipSuccess) e = launch_count(F->mesh_dev, n, F->max_edge, F->counts, s);
-    if (e == hipSuccess) e = launch_scan(F->counts, n, F->bases, F->total, F->overflow, s);
+    // the scan rewrites the order array (the fill has consumed it) as the
+    // emission order, largest chunks first
+    F->order_dev = (int32_t *)(base + off_order);
+    if (e == hipSuccess)
+        e = launch_scan(F->counts, n, F->bases, F->total, F->overflow, F->order_dev, s);
     unsigned char *back = sl.pinned + sl.pinned_bytes - 256;
     if (e == hipSuccess) e = hipMemcpyAsync(back, F->total, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipEventRecord(sl.ev1, s);

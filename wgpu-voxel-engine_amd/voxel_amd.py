@@ -162,6 +162,7 @@ _SIGS = {
     "vx_batch_destroy": (C.c_int, [P]),
     "vx_batch_run": (C.c_int, [P, P]),
     "vx_batch_set_mode": (C.c_int, [P, C.c_int32]),
+    "vx_batch_set_grid": (C.c_int, [P, C.c_int32]),
     "vx_batch_get_mode": (C.c_int, [P, C.POINTER(C.c_int32)]),
     "vx_batch_run_per_chunk": (C.c_int, [P, P]),
     "vx_batch_wait": (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
@@ -507,6 +508,10 @@ class Batch:
 
     def set_mode(self, mode: int):
         self.ctx.check(self.ctx.L.vx_batch_set_mode(self.h, mode), "vx_batch_set_mode")
+
+    def set_grid(self, grid: int):
+        """Workgroups of the fused launch (0 = one per chunk); staged ignores it."""
+        self.ctx.check(self.ctx.L.vx_batch_set_grid(self.h, grid), "vx_batch_set_grid")
 
     @property
     def mode(self) -> int:
