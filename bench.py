@@ -31,7 +31,12 @@ import numpy as np  # noqa: E402
 import voxel_amd as V  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
-FP32_PEAK_TFLOPS = 157.3    # MI355X published fp32 peak
+FP32_PEAK_TFLOPS = 157.3    # MI355X published fp32 peak (packed FP32: 2 FMAs per lane per cycle)
+# 256 CUs x 4 SIMDs x 16 lanes x 2 flops (FMA) x 2.4 GHz: the FP32 rate without packed
+# FP32 instructions, which this build does not issue
+FP32_UNPACKED_TFLOPS = 78.64
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_uniform4096x32.txt")
 GRID = 64                   # 64 x 64 chunks
 EDGE = 32
 # FP32 operations of one restated 3-D Improved Noise evaluation inside fBm
@@ -163,7 +168,24 @@ def frame_leg(ctx, faces_per_chunk: float, frames: int = 20):
             "visible_last_frame": visible, "form": form}
 
 
-def serving_leg(rates=(2000, 20000), duration=1.5):
+def measured_traffic(kernel: str, faces: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was measured on this same workload (same face
+    count); None otherwise."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    if not any(f"{faces} faces" in ln for ln in lines if ln.startswith("#")):
+        return None
+    for ln in lines:
+        parts = ln.split()
+        if len(parts) >= 2 and parts[0] == "kernel" and parts[1] == kernel:
+            return int(dict(p.split("=") for p in parts[2:])["bytes"])
+    return None
+
+
+def serving_leg(rates=(2000, 5000, 10000, 20000), duration=1.5):
     """Streaming (config 4): camera random walk at 20 blocks/s (seed 3),
     render radius 16 chunks of 32^3, Poisson arrivals, sizes 16/32/64
     uniform, 8 GB HBM admission budget with LRU eviction."""
@@ -304,13 +326,19 @@ def main():
         ach = fill_flops / (stage_ms[0] * 1e-3) / 1e12
         roof = {"kernel": "fill_kernel<32> (noise+fill)", "bound": "valu", "achieved": round(ach, 2),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
-                "traffic": None, "ms": round(float(stage_ms[0]), 4)}
+                "traffic": measured_traffic("fill_kernel<32>", total_faces),
+                "ms": round(float(stage_ms[0]), 4),
+                "unpacked_peak": FP32_UNPACKED_TFLOPS,
+                "unpacked_frac": round(ach / FP32_UNPACKED_TFLOPS, 4)}
     else:
         ach = emit_bytes / (stage_ms[3] * 1e-3) / 1e9
         roof = {"kernel": "emit_kernel<32> (cull+compact+emit)", "bound": "hbm",
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("emit_kernel<32>", total_faces),
                 "ms": round(float(stage_ms[3]), 4)}
+    stages["fill"]["traffic"] = measured_traffic("fill_kernel<32>", total_faces)
+    stages["emit"]["traffic"] = measured_traffic("emit_kernel<32>", total_faces)
 
     plain = None
     if not args.no_plain:
