@@ -199,6 +199,31 @@ int vx_frustum_mask(vx_ctx *ctx, const vx_frustum *f, const vx_coord *coords, in
 int vx_frustum_mask_device(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
                            int32_t n, int32_t edge, uint8_t *out_dev);
 
+/* ---- draw lists (src/world.rs:34-38 + ChunkMesh::render, src/chunk.rs:268-277) ----
+ * World::render issues ChunkMesh::render's draw_indexed(0..6, 0, 0..instances)
+ * for every resident mesh whose chunk passes Frustum::contains.  With the
+ * meshes in one arena (mesh i = faces[i] records starting at first[i]) that
+ * loop becomes an indirect draw list, built on the GPU in mesh order:
+ *   VX_DRAW_INSTANCED  InstanceRaw arena:   {6, faces, 0, 0, first}
+ *   VX_DRAW_EXPANDED   vertex/index arena:  {6*faces, 1, 6*first, 4*first, 0}
+ * One entry per visible mesh (zero-face meshes give no-op draws, as in the
+ * reference).  count_dev[0] = number of draws; count_dev[1] != 0 if some
+ * offset did not fit the 32-bit fields.  Asynchronous on the context stream;
+ * all pointers are device pointers; args_dev holds n entries. */
+typedef struct vx_draw_indexed {
+    uint32_t index_count;
+    uint32_t instance_count;
+    uint32_t first_index;
+    int32_t base_vertex;
+    uint32_t first_instance;
+} vx_draw_indexed;
+#define VX_DRAW_INSTANCED 1u
+#define VX_DRAW_EXPANDED 2u
+int vx_frustum_draw_list(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
+                         const uint32_t *faces_dev, const uint64_t *first_dev, int32_t n,
+                         int32_t edge, uint32_t mode, vx_draw_indexed *args_dev,
+                         uint32_t *count_dev);
+
 /* ---- Batches: many Chunk::new + create_mesh jobs in one launch sequence --- */
 /* One request: chunk coord, edge (16, 32 or 64), terrain, and the neighbour
  * snapshot as indices into the same batch (-1 = absent). */
@@ -258,6 +283,11 @@ int vx_batch_read_ids(vx_ctx *ctx, vx_batch *batch, int32_t i, uint8_t *dst, uin
  * request order; indices are chunk-local). */
 int vx_batch_read_mesh(vx_ctx *ctx, vx_batch *batch, uint32_t format, void *dst, uint64_t cap);
 int vx_batch_device_buffers(vx_batch *batch, void **instances, void **vertices, void **indices);
+/* Draw list over the batch's meshes (vx_frustum_draw_list with the batch's
+ * coordinates, face counts and first faces; uniform edge; the mode must
+ * match a format the batch writes).  After a run. */
+int vx_batch_draw_list(vx_ctx *ctx, vx_batch *batch, const vx_frustum *f, uint32_t mode,
+                       vx_draw_indexed *args_dev, uint32_t *count_dev);
 /* Times `reps` replays of each stage with HIP events on the context stream.
  * Staged form: ms[0..3] = average fill, count, scan, emit kernel time;
  * fused form: ms[0] = fused kernel (with its control reset), ms[1..3] = 0;
@@ -381,6 +411,53 @@ typedef struct vx_sim_result {
 } vx_sim_result;
 
 int vx_serve_simulate(vx_ctx *ctx, const vx_sim_desc *desc, vx_sim_result *result);
+
+/* ---- World + ChunkGenerator, tick for tick -------------------------------
+ * The reference's scheduler (ChunkGenerator::build_chunks,
+ * src/chunk_builder.rs:45-352) and world maps (World, src/world.rs:14-58)
+ * with the same per-tick steps: primary ring of radius 1 when the player
+ * enters a new chunk, frustum-tested rings 2..render_distance-1 at y = 0,
+ * one queued load and (when nothing is loading) one rebuild submitted per
+ * tick, one finished chunk and one rebuilt mesh received per tick, rebuilds
+ * queued for the resident neighbours of every arrival, the rebuild queue
+ * sorted and de-duplicated, chunks outside the +-render_distance box
+ * unloaded.  Jobs (Chunk::new + create_mesh with the resident-neighbour
+ * snapshot at submission, or create_mesh of a resident chunk) run on the
+ * GPU.  A job's result becomes receivable `result_delay` ticks after
+ * submission, in submission order: one of the schedules the reference's
+ * thread pool can produce, fixed so that a run is reproducible.  Chunk edge
+ * 32 (src/world.rs:21-22). */
+typedef struct vx_world vx_world;
+typedef struct vx_world_desc {
+    int32_t terrain_id;
+    int32_t render_distance;  /* RENDER_DISTANCE, 5 in the reference (src/world.rs:19) */
+    int32_t result_delay;     /* ticks from a job's submission to its receipt (>= 0) */
+    uint32_t formats;         /* VX_FMT_* of the meshes */
+    float player_pos[3];      /* Player::new (src/player.rs:10-14): world units */
+} vx_world_desc;
+typedef struct vx_world_stats {
+    uint64_t tick;
+    int32_t chunks, meshes, rebuild_queue, in_process;
+    uint64_t jobs_loaded, jobs_rebuilt, received_chunks, received_meshes, unloaded;
+    vx_coord player_chunk;
+} vx_world_stats;
+int vx_world_create(vx_ctx *ctx, const vx_world_desc *desc, vx_world **out);
+int vx_world_destroy(vx_world *world);
+/* One World::update (src/engine.rs:65-74): the player's position this tick
+ * (world units; Player::update_pos) and the frustum (Frustum::update). */
+int vx_world_tick(vx_world *world, const float player_pos[3], const vx_frustum *f);
+int vx_world_stats_get(vx_world *world, vx_world_stats *out);
+/* Coordinates of the resident chunks / meshes (sorted by x, y, z), or the
+ * rebuild queue / chunks in process (in queue order).  With cap = 0 only
+ * *n is set. */
+#define VX_WORLD_CHUNKS 1u
+#define VX_WORLD_MESHES 2u
+#define VX_WORLD_REBUILD_QUEUE 3u
+#define VX_WORLD_IN_PROCESS 4u
+int vx_world_list(vx_world *world, uint32_t what, vx_coord *out, int32_t cap, int32_t *n);
+/* The resident chunk and mesh at a coordinate (retained handles, NULL when
+ * absent; pass NULL for the one you do not want). */
+int vx_world_lookup(vx_world *world, vx_coord c, vx_chunk **chunk, vx_mesh **mesh);
 
 #ifdef __cplusplus
 }

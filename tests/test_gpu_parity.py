@@ -376,3 +376,68 @@ def test_frame_new_chunks_beside_resident_grid_with_cull(ctx, oracle_mod, mode):
     R.close()
     cbuf.free()
     mbuf.free()
+
+
+# ------------------------------------------------------------------ draw lists
+def oracle_draws(O, planes, coords, faces, first, edge, mode):
+    """World::render's loop (world.rs:34-38): one draw per mesh whose chunk
+    passes Frustum::contains, in mesh order, as indirect arguments."""
+    mask = O.frustum_contains(planes, coords, edge).astype(bool)
+    f = np.asarray(faces, np.uint64)[mask]
+    s = np.asarray(first, np.uint64)[mask]
+    out = np.zeros(int(mask.sum()), V.DRAW_DTYPE)
+    if mode == V.DRAW_INSTANCED:
+        out["index_count"], out["instance_count"], out["first_instance"] = 6, f, s
+    else:
+        out["index_count"], out["instance_count"] = 6 * f, 1
+        out["first_index"], out["base_vertex"] = 6 * s, (4 * s).astype(np.int32)
+    return out
+
+
+@pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 70000])
+@pytest.mark.parametrize("mode", [V.DRAW_INSTANCED, V.DRAW_EXPANDED])
+def test_draw_list_matches_oracle(ctx, oracle_mod, n, mode):
+    import test_oracle as T
+    say("draw list", n, mode)
+    rng = np.random.default_rng(n + 7 * mode)
+    eye = rng.uniform(-300, 300, 3)
+    f = V.Frustum(T.colmajor(T.perspective(np.radians(70), 1.6, 0.1, 1024) @
+                             T.look_at(eye, eye + rng.normal(size=3))))
+    coords = rng.integers(-40, 40, size=(n, 3)).astype(np.int32)
+    faces = rng.integers(0, 6000, size=n).astype(np.uint32)
+    first = np.concatenate([[0], np.cumsum(faces, dtype=np.uint64)[:-1]]).astype(np.uint64) if n else \
+        np.zeros(0, np.uint64)
+    got = f.draw_list(ctx, coords, faces, first, 32, mode)
+    ref = oracle_draws(oracle_mod, f.planes, coords, faces, first, 32, mode)
+    assert got.tobytes() == ref.tobytes()
+    if n >= 1024:
+        assert 0 < len(got) < n
+
+
+def test_draw_list_reports_offsets_beyond_32_bits(ctx):
+    f = V.Frustum(np.eye(4, dtype=np.float32).reshape(16))  # clip-space cube
+    coords = np.zeros((1, 3), np.int32)
+    with pytest.raises(V.VoxelError):
+        f.draw_list(ctx, coords, np.array([5], np.uint32), np.array([2**32 - 3], np.uint64), 1,
+                    V.DRAW_INSTANCED)
+
+
+@pytest.mark.parametrize("mode", [V.DRAW_INSTANCED, V.DRAW_EXPANDED])
+def test_batch_draw_list(ctx, oracle_mod, mode):
+    """A resident grid's culled draw list references exactly the visible
+    chunks' slices of the batch arena."""
+    import test_oracle as T
+    coords = [(x, 0, z) for x in range(8) for z in range(8)]
+    b = V.Batch(ctx, coords, 32, ctx.terrain(C5), V.grid_neighbours(8, 8),
+                formats=V.FMT_VERTICES | V.FMT_INSTANCES)
+    b.run()
+    b.wait()
+    eye = np.array([-20.0, 40.0, 120.0])
+    f = V.Frustum(T.colmajor(T.perspective(np.radians(70), 1.5, 0.1, 1024) @
+                             T.look_at(eye, eye + np.array([1.0, -0.4, 0.2]))))
+    fc = [b.chunk_faces(i) for i in range(len(coords))]
+    got = b.draw_list(f, mode)
+    ref = oracle_draws(oracle_mod, f.planes, np.array(coords, np.int32), [c for _, c in fc],
+                       [s for s, _ in fc], 32, mode)
+    assert got.tobytes() == ref.tobytes()
+    assert 0 < len(got) < len(coords)

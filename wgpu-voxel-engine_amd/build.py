@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvoxel_hip.so")
-SOURCES = ["vx_kernels.hip", "vx_api.cpp", "vx_batch.cpp", "vx_server.cpp"]
+SOURCES = ["vx_kernels.hip", "vx_api.cpp", "vx_batch.cpp", "vx_server.cpp", "vx_world.cpp"]
 ARCH = os.environ.get("VX_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -857,6 +857,48 @@ extern "C" int vx_frustum_mask_device(vx_ctx *ctx, const vx_frustum *f, const in
     return VX_OK;
 }
 
+int vx_draw_list_locked(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
+                        const uint32_t *faces_dev, const uint64_t *first_dev, int32_t n,
+                        int32_t edge, uint32_t mode, vx_draw_indexed *args_dev,
+                        uint32_t *count_dev) {
+    const int32_t nb = draw_blocks(n);
+    if (nb > ctx->draw_counts_cap) {
+        if (ctx->draw_counts) {
+            VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // earlier lists may still read it
+            ctx->dev_free(ctx->draw_counts);
+            ctx->draw_counts = nullptr;
+            ctx->draw_counts_cap = 0;
+        }
+        const int32_t cap = std::max(nb, 256);
+        int st = ctx->dev_alloc((size_t)cap * 4, (void **)&ctx->draw_counts);
+        if (st) return st;
+        ctx->draw_counts_cap = cap;
+    }
+    Planes P;
+    memcpy(P.p, f->planes, sizeof(P.p));
+    VX_HIP(ctx, launch_draw_list(P, coords_dev, faces_dev, first_dev, n, edge, mode,
+                                 ctx->draw_counts, reinterpret_cast<DrawArgs *>(args_dev),
+                                 count_dev, ctx->stream));
+    return VX_OK;
+}
+
+extern "C" int vx_frustum_draw_list(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
+                                    const uint32_t *faces_dev, const uint64_t *first_dev,
+                                    int32_t n, int32_t edge, uint32_t mode,
+                                    vx_draw_indexed *args_dev, uint32_t *count_dev) {
+    if (!ctx || !f || n < 0 || edge <= 0 || !count_dev ||
+        (mode != VX_DRAW_INSTANCED && mode != VX_DRAW_EXPANDED) ||
+        (n && (!coords_dev || !faces_dev || !first_dev || !args_dev)))
+        return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!n) {
+        VX_HIP(ctx, hipMemsetAsync(count_dev, 0, 8, ctx->stream));
+        return VX_OK;
+    }
+    return vx_draw_list_locked(ctx, f, coords_dev, faces_dev, first_dev, n, edge, mode, args_dev,
+                               count_dev);
+}
+
 extern "C" int vx_frustum_mask(vx_ctx *ctx, const vx_frustum *f, const vx_coord *coords, int32_t n,
                                int32_t edge, uint8_t *out) {
     if (!ctx || !f || n < 0 || edge <= 0 || (n && (!coords || !out))) return VX_ERR_INVALID;

@@ -179,6 +179,7 @@ void free_batch(vx_batch *b) {
     ctx->dev_free(b->fill_dev);
     ctx->dev_free(b->mesh_dev);
     ctx->dev_free(b->order_dev);
+    ctx->dev_free(b->coords_dev);
     ctx->dev_free(b->ctl);
     ctx->dev_free(b->arena);
     ctx->dev_free(b->fused_dev);
@@ -643,6 +644,33 @@ extern "C" int vx_batch_set_cull(vx_batch *b, const vx_frustum *f, const int32_t
         drop_graph(b);  // kernel arguments change; planes alone are read from memory
     b->cull = job;
     return VX_OK;
+}
+
+extern "C" int vx_batch_draw_list(vx_ctx *ctx, vx_batch *b, const vx_frustum *f, uint32_t mode,
+                                  vx_draw_indexed *args_dev, uint32_t *count_dev) {
+    if (!ctx || !b || b->ctx != ctx || !f || !args_dev || !count_dev) return VX_ERR_INVALID;
+    if (!((mode == VX_DRAW_INSTANCED && (b->formats & kFmtInstances)) ||
+          (mode == VX_DRAW_EXPANDED && (b->formats & kFmtVertices))))
+        return VX_ERR_INVALID;
+    for (const vx_batch_req &r : b->reqs)
+        if (r.edge != b->max_edge) return VX_ERR_INVALID;  // one AABB edge per list
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!b->ran) return VX_ERR_NOT_READY;
+    if (!b->coords_dev) {
+        std::vector<int32_t> c(3 * (size_t)b->n);
+        for (int32_t i = 0; i < b->n; i++) {
+            c[3 * i] = b->reqs[i].coord.x;
+            c[3 * i + 1] = b->reqs[i].coord.y;
+            c[3 * i + 2] = b->reqs[i].coord.z;
+        }
+        int st = ctx->dev_alloc(c.size() * 4, (void **)&b->coords_dev);
+        if (st) return st;
+        VX_HIP(ctx, hipMemcpyAsync(b->coords_dev, c.data(), c.size() * 4, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // c is pageable and goes out of scope
+    }
+    return vx_draw_list_locked(ctx, f, b->coords_dev, b->counts, b->bases, b->n, b->max_edge, mode,
+                               args_dev, count_dev);
 }
 
 extern "C" int vx_batch_device_buffers(vx_batch *b, void **inst, void **vert, void **idx) {

@@ -61,6 +61,8 @@ struct vx_ctx {
     void *scratch = nullptr;    // staging for host-array entry points
     size_t scratch_bytes = 0;
     unsigned char *pinned = nullptr;  // 4 KiB pinned host staging for readbacks
+    uint32_t *draw_counts = nullptr;  // per-workgroup counts of draw-list passes
+    int32_t draw_counts_cap = 0;
 
     void set_error(hipError_t e, const char *what);
     int dev_alloc(size_t n, void **out);
@@ -115,6 +117,7 @@ struct vx_batch {
     vx::FillWork *fill_dev = nullptr;
     vx::MeshWork *mesh_dev = nullptr;
     int32_t *order_dev = nullptr;   // longest-first schedule (null when uniform)
+    int32_t *coords_dev = nullptr;  // chunk coordinates (draw lists), made on first use
     void *ctl = nullptr;            // counts[n] | bases[n] | total | overflow | emit_order[n]
     int32_t *emit_order = nullptr;  // emission order written by the scan (largest first)
     uint32_t *counts = nullptr;
@@ -142,5 +145,10 @@ struct vx_batch {
 };
 
 void vx_batch_unref(vx_batch *b);
+// Frustum-culled indirect draw list over n meshes (callers hold the context lock).
+int vx_draw_list_locked(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
+                        const uint32_t *faces_dev, const uint64_t *first_dev, int32_t n,
+                        int32_t edge, uint32_t mode, vx_draw_indexed *args_dev,
+                        uint32_t *count_dev);
 void vx_chunk_destroy_locked(vx_chunk *c);
 void vx_shared_unref_locked(vx_ctx *ctx, vx::SharedBlock *s);

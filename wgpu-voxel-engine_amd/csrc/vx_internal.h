@@ -133,6 +133,15 @@ struct Planes {
     float p[24];
 };
 
+// Indirect draw arguments (the layout of vx_draw_indexed in include/voxel.h
+// and of WebGPU's drawIndexedIndirect buffer).
+struct DrawArgs {
+    uint32_t index_count, instance_count, first_index;
+    int32_t base_vertex;
+    uint32_t first_instance;
+};
+enum : uint32_t { kDrawInstanced = 1u, kDrawExpanded = 2u };
+
 inline int words_per_row(int edge) { return (edge + 31) / 32; }
 
 // ---- kernel launchers (vx_kernels.hip) --------------------------------
@@ -159,5 +168,10 @@ hipError_t launch_cull(const CullJob &job, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n, int32_t edge,
                           uint8_t *mask, hipStream_t s);
+// Workgroups (and block_counts entries) a draw list of n meshes uses.
+int32_t draw_blocks(int32_t n);
+hipError_t launch_draw_list(const Planes &planes, const int32_t *coords, const uint32_t *faces,
+                            const uint64_t *first, int32_t n, int32_t edge, uint32_t mode,
+                            uint32_t *block_counts, DrawArgs *args, uint32_t *count, hipStream_t s);
 
 }  // namespace vx

@@ -470,7 +470,7 @@ constexpr int kStageWords = 64 * 20;
 template <int MAXN> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
-    uint32_t desc[kDescTile];
+    uint32_t desc[kDescTile / 2];  // 34 KB in all: 4 workgroups per CU
     float P[6][2][4][3];
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
@@ -654,7 +654,7 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
 // scan of the per-thread counts gives every row its first face index, so the
 // face order is (y, z, x, direction) exactly as chunk.rs:65-240 pushes them.
 // Faces are staged as 32-bit descriptors (row | x << 12 | dir << 18) in LDS
-// tiles of kDescTile, then one lane per face writes its records.
+// tiles (the size of S.desc), then one lane per face writes its records.
 template <int N, int MAXN, class SH>
 __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
                           SH &S) {
@@ -681,8 +681,9 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
 
     const bool want_v = (out.formats & kFmtVertices) != 0;
     const bool want_i = (out.formats & kFmtInstances) != 0;
-    for (uint32_t t0 = 0; t0 < F; t0 += kDescTile) {
-        const uint32_t t1 = min(F, t0 + (uint32_t)kDescTile);
+    constexpr uint32_t TILE = sizeof(S.desc) / sizeof(S.desc[0]);
+    for (uint32_t t0 = 0; t0 < F; t0 += TILE) {
+        const uint32_t t1 = min(F, t0 + TILE);
         uint32_t f = first;
         if (f < t1 && f + mine > t0) {
 #pragma unroll 1
@@ -1059,26 +1060,85 @@ __global__ void __launch_bounds__(kBlock) frustum_kernel(Planes planes,
                                                          const int32_t *__restrict__ coords,
                                                          int32_t n, int32_t edge,
                                                          uint8_t *__restrict__ mask) {
-    const float *pl = planes.p;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float px = (float)(coords[3 * i] * edge), py = (float)(coords[3 * i + 1] * edge),
-                pz = (float)(coords[3 * i + 2] * edge);
-    const float e = (float)edge;
-    const float xs[2] = {px, px + e}, ys[2] = {py, py + e}, zs[2] = {pz, pz + e};
-    uint32_t any = 0;
+    mask[i] = aabb_visible(planes.p, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2], edge);
+}
+
+// --------------------------------------------------------------- draw lists
+// World::render (world.rs:34-38) calls ChunkMesh::render (chunk.rs:268-277)
+// -- draw_indexed(0..6, 0, 0..instances) on the mesh's own instance buffer --
+// for every resident mesh whose chunk passes Frustum::contains.  Here the
+// meshes live in one arena (mesh i = faces[i] records from first[i]) and
+// the draws become indirect arguments: one pass counts visible meshes per
+// workgroup, a second gives each workgroup its offset (the sum of the counts
+// before it) and writes the arguments in mesh order.  No atomics; the order
+// is deterministic.
+constexpr int kDrawPer = 4;                     // meshes per thread
+constexpr int kDrawBlock = kBlock * kDrawPer;   // meshes per workgroup
+
+__global__ void __launch_bounds__(kBlock) draw_count_kernel(Planes planes,
+                                                            const int32_t *__restrict__ coords,
+                                                            int32_t n, int32_t edge,
+                                                            uint32_t *__restrict__ block_counts,
+                                                            uint32_t *__restrict__ count) {
+    __shared__ uint32_t red[kBlock / 64];
+    uint32_t c = 0;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const float cx = xs[c & 1], cy = ys[(c >> 2) & 1], cz = zs[(c >> 1) & 1];
-        uint32_t inside = 1;
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const float dist = pl[4 * k] * cx + pl[4 * k + 1] * cy + pl[4 * k + 2] * cz + pl[4 * k + 3];
-            inside &= !(dist < 0.0f);
-        }
-        any |= inside;
+    for (int k = 0; k < kDrawPer; k++) {
+        const int i = blockIdx.x * kDrawBlock + k * kBlock + threadIdx.x;
+        if (i < n) c += aabb_visible(planes.p, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2], edge);
     }
-    mask[i] = (uint8_t)any;
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) {
+        block_counts[blockIdx.x] = c;
+        if (blockIdx.x == 0) count[1] = 0u;  // range flag, set by the write pass
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) draw_write_kernel(Planes planes,
+                                                            const int32_t *__restrict__ coords,
+                                                            const uint32_t *__restrict__ faces,
+                                                            const uint64_t *__restrict__ first,
+                                                            int32_t n, int32_t edge, uint32_t mode,
+                                                            const uint32_t *__restrict__ block_counts,
+                                                            DrawArgs *__restrict__ args,
+                                                            uint32_t *__restrict__ count) {
+    __shared__ uint32_t red[kBlock / 64];
+    const int tid = threadIdx.x;
+    uint32_t before = 0;
+    for (int j = tid; j < (int)blockIdx.x; j += kBlock) before += block_counts[j];
+    const uint32_t off = block_sum(before, red);
+    const int i0 = blockIdx.x * kDrawBlock + tid * kDrawPer;  // contiguous per thread: order kept
+    uint32_t vis = 0, mine = 0;
+#pragma unroll
+    for (int k = 0; k < kDrawPer; k++) {
+        const int i = i0 + k;
+        const uint32_t v = i < n ? aabb_visible(planes.p, coords[3 * i], coords[3 * i + 1],
+                                                coords[3 * i + 2], edge)
+                                 : 0u;
+        vis |= v << k;
+        mine += v;
+    }
+    uint32_t total;
+    uint32_t pos = off + block_exclusive_scan(mine, red, &total);
+    bool range = true;
+#pragma unroll
+    for (int k = 0; k < kDrawPer; k++) {
+        if (!((vis >> k) & 1u)) continue;
+        const uint64_t F = faces[i0 + k], f0 = first[i0 + k];
+        DrawArgs a;
+        if (mode == kDrawExpanded) {  // chunk-local indices: base_vertex moves them to the arena
+            range &= 6 * (f0 + F) <= 0xFFFFFFFFull && 4 * (f0 + F) <= 0x7FFFFFFFull;
+            a = DrawArgs{(uint32_t)(6 * F), 1u, (uint32_t)(6 * f0), (int32_t)(4 * f0), 0u};
+        } else {                      // ChunkMesh::render: 6 indices, F instances
+            range &= f0 + F <= 0xFFFFFFFFull;
+            a = DrawArgs{6u, (uint32_t)F, 0u, 0, (uint32_t)f0};
+        }
+        args[pos++] = a;
+    }
+    if (!range) atomicOr(count + 1, 1u);
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) count[0] = off + total;
 }
 
 // --------------------------------------------------------------- launchers
@@ -1201,6 +1261,19 @@ hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n
                           uint8_t *mask, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     frustum_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(planes, coords, n, edge, mask);
+    return hipGetLastError();
+}
+
+int32_t draw_blocks(int32_t n) { return (n + kDrawBlock - 1) / kDrawBlock; }
+
+hipError_t launch_draw_list(const Planes &planes, const int32_t *coords, const uint32_t *faces,
+                            const uint64_t *first, int32_t n, int32_t edge, uint32_t mode,
+                            uint32_t *block_counts, DrawArgs *args, uint32_t *count, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int32_t nb = draw_blocks(n);
+    draw_count_kernel<<<nb, kBlock, 0, s>>>(planes, coords, n, edge, block_counts, count);
+    draw_write_kernel<<<nb, kBlock, 0, s>>>(planes, coords, faces, first, n, edge, mode, block_counts,
+                                           args, count);
     return hipGetLastError();
 }
 

@@ -111,6 +111,20 @@ class TuneResult(C.Structure):
                 ("cand_grid", C.c_int32 * 8), ("cand_ms", C.c_float * 8)]
 
 
+class WorldDesc(C.Structure):
+    _fields_ = [("terrain_id", C.c_int32), ("render_distance", C.c_int32),
+                ("result_delay", C.c_int32), ("formats", C.c_uint32),
+                ("player_pos", C.c_float * 3)]
+
+
+class WorldStats(C.Structure):
+    _fields_ = [("tick", C.c_uint64), ("chunks", C.c_int32), ("meshes", C.c_int32),
+                ("rebuild_queue", C.c_int32), ("in_process", C.c_int32),
+                ("jobs_loaded", C.c_uint64), ("jobs_rebuilt", C.c_uint64),
+                ("received_chunks", C.c_uint64), ("received_meshes", C.c_uint64),
+                ("unloaded", C.c_uint64), ("player_chunk", Coord)]
+
+
 class BatchReq(C.Structure):
     _fields_ = [("coord", Coord), ("edge", C.c_int32), ("terrain_id", C.c_int32),
                 ("nb", C.c_int32 * 4)]
@@ -154,6 +168,9 @@ _SIGS = {
     "vx_frustum_contains": (C.c_int, [C.POINTER(FrustumC), Coord, C.c_int32]),
     "vx_frustum_mask": (C.c_int, [P, C.POINTER(FrustumC), P, C.c_int32, C.c_int32, P]),
     "vx_frustum_mask_device": (C.c_int, [P, C.POINTER(FrustumC), P, C.c_int32, C.c_int32, P]),
+    "vx_frustum_draw_list": (C.c_int, [P, C.POINTER(FrustumC), P, P, P, C.c_int32, C.c_int32,
+                                       C.c_uint32, P, P]),
+    "vx_batch_draw_list": (C.c_int, [P, P, C.POINTER(FrustumC), C.c_uint32, P, P]),
     "vx_batch_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
     "vx_batch_create_ext": (C.c_int, [P, P, C.c_int32, P, C.c_int32, C.c_uint32, C.c_uint64,
                                       C.POINTER(P)]),
@@ -181,6 +198,12 @@ _SIGS = {
     "vx_server_lookup": (C.c_int, [P, Coord, C.c_int32, C.POINTER(P), C.POINTER(P)]),
     "vx_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "vx_serve_simulate": (C.c_int, [P, C.POINTER(SimDesc), C.POINTER(SimResult)]),
+    "vx_world_create": (C.c_int, [P, C.POINTER(WorldDesc), C.POINTER(P)]),
+    "vx_world_destroy": (C.c_int, [P]),
+    "vx_world_tick": (C.c_int, [P, P, C.POINTER(FrustumC)]),
+    "vx_world_stats_get": (C.c_int, [P, C.POINTER(WorldStats)]),
+    "vx_world_list": (C.c_int, [P, C.c_uint32, P, C.c_int32, C.POINTER(C.c_int32)]),
+    "vx_world_lookup": (C.c_int, [P, Coord, C.POINTER(P), C.POINTER(P)]),
 }
 
 
@@ -426,6 +449,22 @@ class ChunkMesh:
         return out
 
 
+# vx_draw_indexed: one indirect indexed draw (WebGPU drawIndexedIndirect layout)
+DRAW_DTYPE = np.dtype([("index_count", "<u4"), ("instance_count", "<u4"), ("first_index", "<u4"),
+                       ("base_vertex", "<i4"), ("first_instance", "<u4")])
+DRAW_INSTANCED, DRAW_EXPANDED = 1, 2
+
+
+def _read_draws(ctx: "Context", args: "DeviceBuffer", cnt: "DeviceBuffer"):
+    c = cnt.download(np.uint32, 2)
+    if c[1]:
+        raise VoxelError(ERR_INVALID, "draw list: an offset does not fit 32 bits")
+    n = int(c[0])
+    if n == 0:
+        return np.zeros(0, DRAW_DTYPE)
+    return args.download(np.uint8, n * DRAW_DTYPE.itemsize).view(DRAW_DTYPE)
+
+
 class Frustum:
     """Frustum (src/frustum_culling.rs): planes from a view-projection matrix."""
 
@@ -452,6 +491,27 @@ class Frustum:
         ctx.check(ctx.L.vx_frustum_mask(ctx.h, C.byref(self.c), _p(coords), len(coords), edge,
                                         _p(out)), "vx_frustum_mask")
         return out
+
+    def draw_list(self, ctx: Context, coords: np.ndarray, faces: np.ndarray, first: np.ndarray,
+                  edge: int = 32, mode: int = DRAW_INSTANCED) -> np.ndarray:
+        """World::render's culled draw loop as indirect draw arguments (device-built)."""
+        coords = np.ascontiguousarray(np.asarray(coords, np.int32).reshape(-1, 3))
+        n = len(coords)
+        bufs = []
+        if n:
+            bufs = [DeviceBuffer.of(ctx, coords),
+                    DeviceBuffer.of(ctx, np.ascontiguousarray(faces, np.uint32)),
+                    DeviceBuffer.of(ctx, np.ascontiguousarray(first, np.uint64))]
+        ptrs = [b.ptr for b in bufs] if n else [None, None, None]
+        args, cnt = DeviceBuffer(ctx, max(1, n) * DRAW_DTYPE.itemsize), DeviceBuffer(ctx, 8)
+        try:
+            ctx.check(ctx.L.vx_frustum_draw_list(ctx.h, C.byref(self.c), *ptrs, n, edge, mode,
+                                                 args.ptr if n else None, cnt.ptr),
+                      "vx_frustum_draw_list")
+            return _read_draws(ctx, args, cnt)
+        finally:
+            for b in bufs + [args, cnt]:
+                b.free()
 
 
 class Batch:
@@ -508,6 +568,18 @@ class Batch:
 
     def set_mode(self, mode: int):
         self.ctx.check(self.ctx.L.vx_batch_set_mode(self.h, mode), "vx_batch_set_mode")
+
+    def draw_list(self, frustum: "Frustum", mode: int = DRAW_EXPANDED) -> np.ndarray:
+        """Culled indirect draw list over this batch's meshes (after a run)."""
+        args, cnt = DeviceBuffer(self.ctx, self.n * DRAW_DTYPE.itemsize), DeviceBuffer(self.ctx, 8)
+        try:
+            self.ctx.check(self.ctx.L.vx_batch_draw_list(self.ctx.h, self.h, C.byref(frustum.c),
+                                                         mode, args.ptr, cnt.ptr),
+                           "vx_batch_draw_list")
+            return _read_draws(self.ctx, args, cnt)
+        finally:
+            args.free()
+            cnt.free()
 
     def set_grid(self, grid: int):
         """Workgroups of the fused launch (0 = one per chunk); staged ignores it."""
@@ -575,6 +647,65 @@ class Batch:
 
 
 NB_NONE, NB_RESIDENT = 0, 1
+
+
+WORLD_CHUNKS, WORLD_MESHES, WORLD_REBUILD_QUEUE, WORLD_IN_PROCESS = 1, 2, 3, 4
+
+
+class World:
+    """The reference's World + ChunkGenerator stepped tick by tick
+    (src/world.rs:14-58, src/chunk_builder.rs:45-352), jobs on the GPU."""
+
+    def __init__(self, ctx: Context, terrain_id: int, player_pos, render_distance: int = 5,
+                 result_delay: int = 1, formats: int = FMT_VERTICES):
+        d = WorldDesc(terrain_id, render_distance, result_delay, formats,
+                      (C.c_float * 3)(*map(float, player_pos)))
+        h = P()
+        ctx.check(ctx.L.vx_world_create(ctx.h, C.byref(d), C.byref(h)), "vx_world_create")
+        self.ctx, self.h, self.formats = ctx, h, formats
+
+    def close(self):
+        if self.h:
+            self.ctx.L.vx_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                self.close()
+        except Exception:
+            pass
+
+    def tick(self, player_pos, frustum: "Frustum"):
+        p = np.ascontiguousarray(np.asarray(player_pos, np.float32).reshape(3))
+        self.ctx.check(self.ctx.L.vx_world_tick(self.h, _p(p), C.byref(frustum.c)), "vx_world_tick")
+
+    def stats(self) -> dict:
+        s = WorldStats()
+        self.ctx.check(self.ctx.L.vx_world_stats_get(self.h, C.byref(s)), "vx_world_stats_get")
+        out = {n: getattr(s, n) for n, _ in WorldStats._fields_ if n != "player_chunk"}
+        out["player_chunk"] = (s.player_chunk.x, s.player_chunk.y, s.player_chunk.z)
+        return out
+
+    def list(self, what: int = WORLD_CHUNKS) -> list:
+        n = C.c_int32()
+        self.ctx.check(self.ctx.L.vx_world_list(self.h, what, None, 0, C.byref(n)), "vx_world_list")
+        arr = (Coord * max(1, n.value))()
+        self.ctx.check(self.ctx.L.vx_world_list(self.h, what, C.cast(arr, P), n.value, C.byref(n)),
+                       "vx_world_list")
+        return [(arr[i].x, arr[i].y, arr[i].z) for i in range(n.value)]
+
+    def mesh(self, coord):
+        m = P()
+        self.ctx.check(self.ctx.L.vx_world_lookup(self.h, Coord(*coord), None, C.byref(m)),
+                       "vx_world_lookup")
+        return ChunkMesh(self.ctx, m, self.formats) if m else None
+
+    def chunk(self, coord):
+        c = P()
+        self.ctx.check(self.ctx.L.vx_world_lookup(self.h, Coord(*coord), C.byref(c), None),
+                       "vx_world_lookup")
+        return Chunk(self.ctx, c) if c else None
 
 
 class Server:
