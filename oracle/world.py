@@ -26,9 +26,8 @@ def to_chunk_coord(pos):
 
 
 class WorldModel:
-    def __init__(self, terrain, player_pos, planes_of, render_distance=5, result_delay=1):
-        """terrain: oracle Terrain; planes_of(tick) -> 24 plane floats of that
-        tick's frustum (only needed through frustum_contains)."""
+    def __init__(self, terrain, player_pos, render_distance=5, result_delay=1):
+        """terrain: oracle Terrain; player_pos: Player::new's position (world units)."""
         self.t = terrain
         self.rd = render_distance
         self.delay = result_delay

@@ -28,7 +28,7 @@ def ring_order(px, pz, rd):
 
 def test_stationary_player_loads_rings_one_per_tick(oracle_mod):
     import world as W
-    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), None, render_distance=5,
+    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), render_distance=5,
                      result_delay=1)
     order = ring_order(0, 0, 5)
     assert len(order) == 8 * (2 + 3 + 4)
@@ -45,7 +45,7 @@ def test_stationary_player_loads_rings_one_per_tick(oracle_mod):
 
 def test_entering_a_chunk_loads_the_primary_ring_in_order(oracle_mod):
     import world as W
-    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), None, render_distance=2,
+    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), render_distance=2,
                      result_delay=3)
     m.tick((48.0, 40.0, 16.0), ALL)  # into chunk x = 1
     assert m.player_chunk == (1, 1, 0)
@@ -62,7 +62,7 @@ def test_entering_a_chunk_loads_the_primary_ring_in_order(oracle_mod):
 
 def test_arrivals_queue_sorted_deduplicated_rebuilds(oracle_mod):
     import world as W
-    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), None, render_distance=2,
+    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), render_distance=2,
                      result_delay=1)
     m.tick((48.0, 40.0, 16.0), ALL)  # 9 primary jobs
     seen = []
@@ -84,7 +84,7 @@ def test_arrivals_queue_sorted_deduplicated_rebuilds(oracle_mod):
 
 def test_unload_box_and_stale_rebuilds(oracle_mod):
     import world as W
-    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), None, render_distance=2,
+    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), render_distance=2,
                      result_delay=0)
     m.tick((48.0, 40.0, 16.0), ALL)
     for _ in range(12):
@@ -99,7 +99,7 @@ def test_unload_box_and_stale_rebuilds(oracle_mod):
 
 def test_delay_zero_receives_in_the_same_tick(oracle_mod):
     import world as W
-    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), None, render_distance=3,
+    m = W.WorldModel(small_terrain(oracle_mod), (16.0, 40.0, 16.0), render_distance=3,
                      result_delay=0)
     m.tick((16.0, 40.0, 16.0), ALL)
     assert m.received_chunks == 1 and m.in_process == []
