@@ -17,6 +17,7 @@ Prints ONE JSON line on stdout.  Progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import statistics
@@ -154,6 +155,21 @@ def frame_leg(ctx, faces_per_chunk: float, frames: int = 20):
     assert B.wait() == new_faces
     visible = int(mbuf.download(np.uint8, len(rc)).sum())
     form = "fused" if B.mode == V.BATCH_FUSED else "staged"
+    # World::render's loop over the resident meshes as an indirect draw list
+    f = camera()
+    args, cnt = V.DeviceBuffer(ctx, len(rc) * V.DRAW_DTYPE.itemsize), V.DeviceBuffer(ctx, 8)
+    draw = lambda: ctx.check(ctx.L.vx_batch_draw_list(ctx.h, R.h, C.byref(f.c), V.DRAW_EXPANDED,
+                                                       args.ptr, cnt.ptr), "vx_batch_draw_list")
+    draw()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        draw()
+    ctx.synchronize()
+    draw_ms = (time.perf_counter() - t0) * 1e3 / frames
+    draws = int(cnt.download(np.uint32, 2)[0])
+    args.free()
+    cnt.free()
     B.close()
     R.close()
     for c in ext:
@@ -165,7 +181,42 @@ def frame_leg(ctx, faces_per_chunk: float, frames: int = 20):
             "resident_chunks_per_s": round(len(rc) / gen_s, 1),
             "new_chunks_per_frame": len(nc), "new_faces_per_frame": new_faces,
             "frame_ms": round(el * 1e3 / frames, 4), "frames": frames,
-            "visible_last_frame": visible, "form": form}
+            "visible_last_frame": visible, "form": form,
+            "draw_list_ms": round(draw_ms, 4), "draws": draws}
+
+
+def world_leg(ticks: int = 130):
+    """World + ChunkGenerator stepped tick by tick (vx_world, jobs on the GPU):
+    a scripted walk (move, stand while the loads drain and rebuilds run, turn,
+    jump far, stand), render distance 5, results receivable one tick after
+    submission, fBm seed 5."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import test_oracle as T  # camera helpers only
+
+    with V.Context(0) as c:
+        t5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64,
+                       lacunarity=2.0, gain=0.5, y_scale=1 / 128)
+        start = np.array([16.0, 40.0, 16.0])
+        w = V.World(c, c.terrain(t5), start, render_distance=5, result_delay=1)
+        pos, look = start.copy(), (7.0, -8.0, 3.5)
+        t0 = time.perf_counter()
+        for k in range(ticks):
+            step = ((7.0, 0.0, 3.0) if k < 25 else (0.0, 0.0, 0.0) if k < 90 else
+                    (-4.0, 0.0, 6.0) if k < 100 else (400.0, 0.0, 0.0) if k == 100 else
+                    (0.0, 0.0, 0.0))
+            if step[0] or step[2]:
+                look = (step[0], -8.0, step[2] + 0.5)
+            pos = pos + np.array(step)
+            m = T.colmajor(T.perspective(np.radians(70), 16 / 9, 0.1, 1024) @
+                           T.look_at(pos, pos + np.array(look)))
+            w.tick(pos.astype(np.float32), V.Frustum(m))
+        c.synchronize()
+        el = time.perf_counter() - t0
+        st = w.stats()
+        w.close()
+    return {"ticks": ticks, "ms_per_tick": round(el * 1e3 / ticks, 4),
+            "jobs_loaded": st["jobs_loaded"], "jobs_rebuilt": st["jobs_rebuilt"],
+            "unloaded": st["unloaded"], "resident_chunks": st["chunks"]}
 
 
 def measured_traffic(kernel: str, faces: int):
@@ -222,6 +273,7 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-frame", action="store_true", help="skip the full-frame leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the streaming leg")
+    ap.add_argument("--no-world", action="store_true", help="skip the tick-by-tick world leg")
     ap.add_argument("--pmc-run", choices=["fused", "staged"], default=None,
                     help="counter-collection run: 2 runs of the batch in one form, then a "
                          "256 MiB plain copy kernel (the byte-counter reference); no timing")
@@ -367,6 +419,11 @@ def main():
     if not args.no_serve:
         serving = serving_leg()
 
+    world = None
+    if not args.no_world:
+        world = world_leg()
+        log(f"[bench] world: {world['ms_per_tick']} ms per tick over {world['ticks']} ticks")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, all host cores)...")
@@ -400,6 +457,7 @@ def main():
         "other_form_gpu": other,
         "full_frame": frame,
         "serving": serving,
+        "world": world,
     }
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)
