@@ -94,6 +94,7 @@ template <int MAXN> struct FillShared {
     int perm[512];
     float4 lut[8][16];
     float4 ytab[kMaxOctaves * MAXN];   // per (octave, y): fy, fy-1, fade(fy), Y cell
+    uint32_t ychg[MAXN];               // per y: bit o set iff octave o's Y cell differs from y-1
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t red[kBlock / 64];
@@ -244,6 +245,110 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
     }
 }
 
+// The same density with the octave loop inside the row loop, for OCT
+// octaves known at compile time: the fBm sum of a voxel is one register (no
+// indexed register moves), the occupancy ballot follows each row, and the
+// per-octave rehash is taken from the row's change mask (one uniform LDS
+// word per row instead of one readfirstlane per octave).  Per-octave column
+// terms (fade weights, x permutation entries, z cell) and the 8 corner
+// coefficients stay in registers; the lattice fractions are recomputed in
+// the rare rehash.  Octaves are summed in the same order as the oracle.
+template <int N, int MAXN, bool DBG, int SEG, int OCT, class SH>
+__device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
+                                  SH &S, float *noise_out) {
+    constexpr int L = 64 / N;
+    constexpr int NSEG = N / SEG;
+    static_assert(N % SEG == 0, "segments tile the column");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int G = (N + H.n + L - 1) / L;
+    float fo[OCT], ao[OCT];
+    {
+        float f = T.frequency, a = 1.0f;
+#pragma unroll
+        for (int o = 0; o < OCT; o++) {
+            fo[o] = f;
+            ao[o] = a;
+            f = f * T.lacunarity;
+            a = a * T.gain;
+        }
+    }
+    for (int item = wave; item < G * NSEG; item += kBlock / 64) {
+        const int g = item / NSEG;
+        const int y0 = (item - g * NSEG) * SEG;
+        const int li = g * L + lane / N;
+        int wxi, wzi;
+        line_column<N>(li, lane, H, ox, oz, wxi, wzi);
+        float u[OCT], wz[OCT], c[OCT][8], gy[OCT][8];
+        int pX0[OCT], pX1[OCT], Zi[OCT];
+#pragma unroll
+        for (int o = 0; o < OCT; o++) {
+            const float X = (float)wxi * fo[o];
+            const float fxl = floorf(X);
+            u[o] = fade(X - fxl);
+            const int Xi = (int)fxl & 255;
+            pX0[o] = S.perm[Xi];
+            pX1[o] = S.perm[Xi + 1];
+            const float Z = (float)wzi * fo[o];
+            const float fzl = floorf(Z);
+            wz[o] = fade(Z - fzl);
+            Zi[o] = (int)fzl & 255;
+        }
+#pragma unroll 1
+        for (int y = y0; y < y0 + SEG; y++) {
+            uint32_t chg = y == y0 ? (1u << OCT) - 1u
+                                   : (uint32_t)__builtin_amdgcn_readfirstlane((int)S.ychg[y]);
+            // Rehash the octaves entering a new y cell (rare: a uniform loop
+            // over the set bits, so the rehash code exists once).
+#pragma unroll 1
+            for (; chg; chg &= chg - 1u) {
+                const int o = __builtin_ctz(chg);
+                const int Yi = __builtin_amdgcn_readfirstlane(__float_as_int(S.ytab[o * N + y].w));
+                float f = T.frequency;
+                for (int k = 0; k < o; k++) f = f * T.lacunarity;
+                const float X = (float)wxi * f, Z = (float)wzi * f;
+                const float fx = X - floorf(X), fz = Z - floorf(Z);
+                const float fx1 = fx - 1.0f, fz1 = fz - 1.0f;
+                int px0 = pX0[0], px1 = pX1[0], zi = Zi[0];
+#pragma unroll
+                for (int q = 1; q < OCT; q++)
+                    if (o == q) px0 = pX0[q], px1 = pX1[q], zi = Zi[q];
+                const int A = px0 + Yi, B = px1 + Yi;
+                const int AA = S.perm[A] + zi, AB = S.perm[A + 1] + zi;
+                const int BA = S.perm[B] + zi, BB = S.perm[B + 1] + zi;
+                const int h[8] = {S.perm[AA],     S.perm[BA],     S.perm[AB],
+                                  S.perm[BB],     S.perm[AA + 1], S.perm[BA + 1],
+                                  S.perm[AB + 1], S.perm[BB + 1]};
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const float4 Lk = S.lut[k][h[k] & 15];
+                    const float xr = (k & 1) ? fx1 : fx;
+                    const float zr = (k & 4) ? fz1 : fz;
+                    const float ck = fmaf(Lk.x, xr, Lk.y * zr);
+#pragma unroll
+                    for (int q = 0; q < OCT; q++)
+                        if (o == q) c[q][k] = ck, gy[q][k] = Lk.z;
+                }
+            }
+            float sum = 0.0f;
+#pragma unroll
+            for (int o = 0; o < OCT; o++) {
+                const float4 t = S.ytab[o * N + y];
+                const float g0 = fmaf(t.x, gy[o][0], c[o][0]), g1 = fmaf(t.x, gy[o][1], c[o][1]);
+                const float g2 = fmaf(t.y, gy[o][2], c[o][2]), g3 = fmaf(t.y, gy[o][3], c[o][3]);
+                const float g4 = fmaf(t.x, gy[o][4], c[o][4]), g5 = fmaf(t.x, gy[o][5], c[o][5]);
+                const float g6 = fmaf(t.y, gy[o][6], c[o][6]), g7 = fmaf(t.y, gy[o][7], c[o][7]);
+                const float n = lerp(wz[o], lerp(t.z, lerp(u[o], g0, g1), lerp(u[o], g2, g3)),
+                                     lerp(t.z, lerp(u[o], g4, g5), lerp(u[o], g6, g7)));
+                sum = sum + ao[o] * n;
+            }
+            const float thr = (float)(oy + y) * T.y_scale;
+            store_lines<N, MAXN>(S, H, __ballot((sum - thr) > 0.0f), y, g, lane);
+            if (DBG && li < N)
+                noise_out[(size_t)(wxi - ox) + (size_t)N * li + (size_t)N * N * y] = sum;
+        }
+    }
+}
+
 // 2-D fBm heightmap for every line: h = floor(height_scale*(n+1)/2) clamped
 // to [0, height_max]; voxel solid iff wy <= h.
 template <int N, int MAXN, bool DBG, class SH>
@@ -285,17 +390,43 @@ __device__ __forceinline__ void build_ytab(const TerrainDev &T, int oy, SH &S) {
         const float fy = Y - fyl;
         S.ytab[i] = make_float4(fy, fy - 1.0f, fade(fy), __int_as_float((int)fyl & 255));
     }
+    // Rows where an octave enters a new lattice cell (the hash uses the cell
+    // index mod 256, which is what is compared); row 0 rehashes everything.
+    for (int y = threadIdx.x; y < N; y += kBlock) {
+        uint32_t m = 0;
+        float f = T.frequency;
+        for (int o = 0; o < T.octaves; o++) {
+            const int a = (int)floorf((float)(oy + y) * f) & 255;
+            const int b = (int)floorf((float)(oy + y - 1) * f) & 255;
+            if (y == 0 || a != b) m |= 1u << o;
+            f = f * T.lacunarity;
+        }
+        S.ychg[y] = m;
+    }
 }
 
 // Fills S.occ (and the recomputed halo planes) for terrain T.
-template <int N, int MAXN, bool DBG, int SEG, class SH>
+// OCT_INNER: the octave-inner density loop for 1..4 octaves (the fill
+// kernels); the fused kernel keeps the octave-outer loop, which needs fewer
+// registers next to its emit phase.
+template <int N, int MAXN, bool DBG, int SEG, bool OCT_INNER, class SH>
 __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
                               float *noise_out) {
     using R = typename RowType<MAXN>::T;
-    if (T.kind == kDensity) {
+    if (T.kind == kDensity && !OCT_INNER) {
         build_ytab<N>(T, oy, S);
         __syncthreads();
         density_lines<N, MAXN, DBG, SEG>(T, ox, oy, oz, H, S, noise_out);
+    } else if (T.kind == kDensity) {
+        build_ytab<N>(T, oy, S);
+        __syncthreads();
+        switch (T.octaves) {
+        case 1: density_lines_oct<N, MAXN, DBG, SEG, 1>(T, ox, oy, oz, H, S, noise_out); break;
+        case 2: density_lines_oct<N, MAXN, DBG, SEG, 2>(T, ox, oy, oz, H, S, noise_out); break;
+        case 3: density_lines_oct<N, MAXN, DBG, SEG, 3>(T, ox, oy, oz, H, S, noise_out); break;
+        case 4: density_lines_oct<N, MAXN, DBG, SEG, 4>(T, ox, oy, oz, H, S, noise_out); break;
+        default: density_lines<N, MAXN, DBG, SEG>(T, ox, oy, oz, H, S, noise_out); break;
+        }
     } else if (T.kind == kHeightmap) {
         heightmap_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
     } else {
@@ -369,7 +500,8 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
 template <int N, int MAXN, bool DBG>
 __device__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
     const Halo H{0, 0u};
-    generate_rows<N, MAXN, DBG, (N < 32 ? N : 32)>(T, w.cx * N, w.cy * N, w.cz * N, H, S, w.noise_out);
+    generate_rows<N, MAXN, DBG, (N < 32 ? N : 32), true>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
+                                                         w.noise_out);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
 }
@@ -860,6 +992,7 @@ template <int MAXN> struct FusedShared {
     int perm[512];
     float4 lut[8][16];
     float4 ytab[kMaxOctaves * MAXN];
+    uint32_t ychg[MAXN];
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t desc[kDescTile];
@@ -913,8 +1046,8 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
         S.nb[k][y] = v;
     }
     // Halo lines make N + 4 lines: shorter y segments keep the waves even.
-    generate_rows<N, MAXN, false, (N == 64 ? 32 : N / 2)>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
-                                                          nullptr);
+    generate_rows<N, MAXN, false, (N == 64 ? 32 : N / 2), false>(T, w.cx * N, w.cy * N, w.cz * N,
+                                                                 H, S, nullptr);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
     const uint32_t F = count_rows<N, MAXN>(S);
