@@ -793,6 +793,23 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
     using R = typename RowType<MAXN>::T;
     constexpr int RPT = (N * N) / kBlock;
     const int tid = threadIdx.x;
+    const bool want_v = (out.formats & kFmtVertices) != 0;
+    const bool want_i = (out.formats & kFmtInstances) != 0;
+    if (want_v) {
+        // Indices first: pure arithmetic of the face count, so their stores
+        // drain while the rows are scanned and the face descriptors built.
+        // quad::INDICES = [0,2,1,3,1,2] + 4*local face; two indices per store
+        uint2 *dst = reinterpret_cast<uint2 *>(out.indices + base * 6);
+        for (uint32_t q = tid; q < 3 * F; q += kBlock) {
+            const uint32_t face = q / 3, part = q - face * 3;
+            const uint32_t b4 = 4 * face;
+            uint2 v;
+            if (part == 0) v = make_uint2(b4 + 0, b4 + 2);
+            else if (part == 1) v = make_uint2(b4 + 1, b4 + 3);
+            else v = make_uint2(b4 + 1, b4 + 2);
+            dst[q] = v;
+        }
+    }
     const int r0 = tid * RPT;
     uint32_t rc[RPT];
     uint32_t mine = 0;
@@ -811,8 +828,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
     const uint32_t first = block_exclusive_scan(mine, S.red, &total);
     (void)total;  // equals F (count kernel); F is authoritative for sizing
 
-    const bool want_v = (out.formats & kFmtVertices) != 0;
-    const bool want_i = (out.formats & kFmtInstances) != 0;
+
     constexpr uint32_t TILE = sizeof(S.desc) / sizeof(S.desc[0]);
     for (uint32_t t0 = 0; t0 < F; t0 += TILE) {
         const uint32_t t1 = min(F, t0 + TILE);
@@ -913,19 +929,6 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             }
         }
         __syncthreads();
-    }
-    if (out.formats & kFmtVertices) {
-        // quad::INDICES = [0,2,1,3,1,2] + 4*local face; two indices per store
-        uint2 *dst = reinterpret_cast<uint2 *>(out.indices + base * 6);
-        for (uint32_t q = tid; q < 3 * F; q += kBlock) {
-            const uint32_t face = q / 3, part = q - face * 3;
-            const uint32_t b4 = 4 * face;
-            uint2 v;
-            if (part == 0) v = make_uint2(b4 + 0, b4 + 2);
-            else if (part == 1) v = make_uint2(b4 + 1, b4 + 3);
-            else v = make_uint2(b4 + 1, b4 + 2);
-            dst[q] = v;
-        }
     }
 }
 
