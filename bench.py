@@ -32,10 +32,9 @@ import numpy as np  # noqa: E402
 import voxel_amd as V  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
-FP32_PEAK_TFLOPS = 157.3    # MI355X published fp32 peak (packed FP32: 2 FMAs per lane per cycle)
-# 256 CUs x 4 SIMDs x 16 lanes x 2 flops (FMA) x 2.4 GHz: the FP32 rate without packed
-# FP32 instructions, which this build does not issue
-FP32_UNPACKED_TFLOPS = 78.64
+FP32_PEAK_TFLOPS = 157.3    # MI355X published fp32 peak
+# VALU issue rate behind that figure: 1024 SIMDs x 32 lanes x 2.4 GHz (one FMA = 2 flops per lane)
+VALU_LANE_OPS_T = 78.64
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_uniform4096x32.txt")
 GRID = 64                   # 64 x 64 chunks
@@ -219,9 +218,10 @@ def world_leg(ticks: int = 130):
             "unloaded": st["unloaded"], "resident_chunks": st["chunks"]}
 
 
-def measured_traffic(kernel: str, faces: int):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it was measured on this same workload (same face
+def measured_counter(kernel: str, faces: int, key: str = "bytes"):
+    """A per-launch figure of `kernel` from the committed rocprofv3 summary
+    (key "bytes": HBM bytes from FETCH_SIZE / WRITE_SIZE; "valu_insts":
+    SQ_INSTS_VALU), when it was measured on this same workload (same face
     count); None otherwise."""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
@@ -232,8 +232,13 @@ def measured_traffic(kernel: str, faces: int):
     for ln in lines:
         parts = ln.split()
         if len(parts) >= 2 and parts[0] == "kernel" and parts[1] == kernel:
-            return int(dict(p.split("=") for p in parts[2:])["bytes"])
+            v = dict(p.split("=") for p in parts[2:]).get(key)
+            return int(v) if v is not None else None
     return None
+
+
+def measured_traffic(kernel: str, faces: int):
+    return measured_counter(kernel, faces, "bytes")
 
 
 def serving_leg(rates=(2000, 5000, 10000, 20000), duration=1.5):
@@ -379,9 +384,12 @@ def main():
         roof = {"kernel": "fill_kernel<32> (noise+fill)", "bound": "valu", "achieved": round(ach, 2),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
                 "traffic": measured_traffic("fill_kernel<32>", total_faces),
-                "ms": round(float(stage_ms[0]), 4),
-                "unpacked_peak": FP32_UNPACKED_TFLOPS,
-                "unpacked_frac": round(ach / FP32_UNPACKED_TFLOPS, 4)}
+                "ms": round(float(stage_ms[0]), 4)}
+        vi = measured_counter("fill_kernel<32>", total_faces, "valu_insts")
+        if vi:  # executed VALU stream against the SIMDs' issue rate
+            lane_ops = vi * 64 / (stage_ms[0] * 1e-3) / 1e12
+            roof["valu_issue"] = {"insts_per_launch": vi, "lane_ops_T_per_s": round(lane_ops, 2),
+                                  "peak": VALU_LANE_OPS_T, "frac": round(lane_ops / VALU_LANE_OPS_T, 4)}
     else:
         ach = emit_bytes / (stage_ms[3] * 1e-3) / 1e9
         roof = {"kernel": "emit_kernel<32> (cull+compact+emit)", "bound": "hbm",
