@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(kBlock) pack_one_kernel(FillWork w) {
 // InstanceRaw each) so that global stores are lane-contiguous: a record per
 // lane written as 16-B pieces at an 80-B stride reaches about half the
 // HBM write rate of contiguous 16-B lanes.
-constexpr int kStageWords = 64 * 20;
+constexpr int kStageWords = 32 * 20;  // 32 records per wave pass: 2.5 KB
 
 template <int MAXN> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
@@ -878,54 +878,67 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             const float t[3] = {(float)wx, (float)wy, (float)wz};
             const int sg[3] = {wx < 0, wy < 0, wz < 0};
             const uint64_t face0 = base + t0 + q0;
+            // the 64 records go out in two passes of 32 (lanes 0-31, then
+            // 32-63), each staged in the wave's 2.5 KB slot
+            const int half = lane >> 5, hl = lane & 31;
             if (want_v) {
-                if (valid) {
-                    float v[20];
+                float v[20];
 #pragma unroll
-                    for (int c = 0; c < 4; c++) {
+                for (int c = 0; c < 4; c++) {
 #pragma unroll
-                        for (int k = 0; k < 3; k++) v[c * 5 + k] = S.P[d][sg[k]][c][k] + t[k];
-                        v[c * 5 + 3] = (c & 1) ? 1.0f : 0.0f;
-                        v[c * 5 + 4] = (c & 2) ? 1.0f : 0.0f;
+                    for (int k = 0; k < 3; k++) v[c * 5 + k] = S.P[d][sg[k]][c][k] + t[k];
+                    v[c * 5 + 3] = (c & 1) ? 1.0f : 0.0f;
+                    v[c * 5 + 4] = (c & 2) ? 1.0f : 0.0f;
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (nv <= 32u * h) break;
+                    const uint32_t hn = min(32u, nv - 32u * h);
+                    if (valid && half == h) {
+#pragma unroll
+                        for (int k = 0; k < 5; k++)
+                            st4[hl * 5 + k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
                     }
+                    wave_sync();
+                    float4 *dst = reinterpret_cast<float4 *>(out.vertices + (face0 + 32u * h) * 20);
 #pragma unroll
-                    for (int k = 0; k < 5; k++)
-                        st4[lane * 5 + k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+                    for (int k = 0; k < 3; k++) {
+                        const uint32_t i = 64u * k + lane;
+                        if (i < 5u * hn) dst[i] = st4[i];
+                    }
+                    wave_sync();
                 }
-                wave_sync();
-                float4 *dst = reinterpret_cast<float4 *>(out.vertices + face0 * 20);
-#pragma unroll
-                for (int k = 0; k < 5; k++) {
-                    const uint32_t i = 64u * k + lane;
-                    if (i < 5u * nv) dst[i] = st4[i];
-                }
-                wave_sync();
             }
             if (want_i) {
-                if (valid) {
-                    uint32_t blk = w.solid_type;
-                    if (blk == kBlockMixed)
-                        blk = w.ids[(size_t)x + (size_t)N * z + (size_t)N * N * y];
-                    uint32_t *rec = stw + lane * 17;
+                uint32_t blk = w.solid_type;
+                if (valid && blk == kBlockMixed)
+                    blk = w.ids[(size_t)x + (size_t)N * z + (size_t)N * N * y];
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
+                for (int h = 0; h < 2; h++) {
+                    if (nv <= 32u * h) break;
+                    const uint32_t hn = min(32u, nv - 32u * h);
+                    if (valid && half == h) {
+                        uint32_t *rec = stw + hl * 17;
 #pragma unroll
-                        for (int k = 0; k < 3; k++) rec[c * 4 + k] = __float_as_uint(S.Mb[d][sg[k]][k][c]);
-                        rec[c * 4 + 3] = 0u;  // M(3,c) = +0
+                        for (int c = 0; c < 3; c++) {
+#pragma unroll
+                            for (int k = 0; k < 3; k++) rec[c * 4 + k] = __float_as_uint(S.Mb[d][sg[k]][k][c]);
+                            rec[c * 4 + 3] = 0u;  // M(3,c) = +0
+                        }
+#pragma unroll
+                        for (int k = 0; k < 3; k++) rec[12 + k] = __float_as_uint(t[k]);
+                        rec[15] = 0x3F800000u;  // 1.0f
+                        rec[16] = S.tex[blk][d];
                     }
+                    wave_sync();
+                    uint32_t *dst = out.instances + (face0 + 32u * h) * 17;
 #pragma unroll
-                    for (int k = 0; k < 3; k++) rec[12 + k] = __float_as_uint(t[k]);
-                    rec[15] = 0x3F800000u;  // 1.0f
-                    rec[16] = S.tex[blk][d];
+                    for (int k = 0; k < 9; k++) {
+                        const uint32_t i = 64u * k + lane;
+                        if (i < 17u * hn) dst[i] = stw[i];
+                    }
+                    wave_sync();
                 }
-                wave_sync();
-                uint32_t *dst = out.instances + face0 * 17;
-#pragma unroll
-                for (int k = 0; k < 17; k++) {
-                    const uint32_t i = 64u * k + lane;
-                    if (i < 17u * nv) dst[i] = stw[i];
-                }
-                wave_sync();
             }
         }
         __syncthreads();
