@@ -972,8 +972,11 @@ __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint
     }
 }
 
+// Register cap: six waves per SIMD (80 VGPRs; the 24 KB of LDS also allows
+// six workgroups per CU), so more workgroups overlap their descriptor and
+// store phases.
 template <int MAXN>
-__global__ void __launch_bounds__(kBlock) emit_kernel(const MeshWork *__restrict__ work,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) emit_kernel(const MeshWork *__restrict__ work,
                                                       const int32_t *__restrict__ order,
                                                       const uint64_t *__restrict__ bases,
                                                       const uint32_t *__restrict__ counts,
