@@ -1,0 +1,149 @@
+"""Parity at the configured full sizes: every chunk's face count against the
+oracle's multithreaded driver, byte-exact meshes for a seeded sample, the
+batch's prefix structure, and full-size cull masks."""
+import os
+
+import numpy as np
+import pytest
+
+import voxel_amd as V
+from test_gpu_parity import C5, bits, oterrain, say
+
+pytestmark = pytest.mark.gpu
+
+C2 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
+               gain=0.5, y_scale=1 / 128)
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def check_sample(O, b, t, coords, edges, nb, idx, verts, idxs):
+    """Byte-exact vertices and indices of chunks `idx` (with their in-batch
+    neighbours) against the oracle."""
+    for i in idx:
+        e = int(edges[i])
+        ids = O.chunk_fill(t, *map(int, coords[i]), e)
+        adj = [O.chunk_fill(t, *map(int, coords[j]), e) if j >= 0 else None for j in nb[i]]
+        faces, _, vert, ix = O.mesh_buffers(ids, e, tuple(int(v) * e for v in coords[i]), adj)
+        first, cnt = b.chunk_faces(int(i))
+        assert cnt == len(faces), i
+        assert (bits(verts[4 * first:4 * (first + cnt)]) == bits(vert)).all(), i
+        assert (idxs[6 * first:6 * (first + cnt)] == ix).all(), i
+
+
+@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED])
+def test_config2_full_grid(ctx, oracle_mod, mode):
+    """4096 chunks of 32^3 (64 x 64 grid, seed 7): every chunk's face count
+    equals the oracle's, bases are the exclusive prefix in request order, and
+    16 seeded chunks match byte for byte."""
+    O = oracle_mod
+    say("config 2 full", mode)
+    coords = np.array([(x, 0, z) for x in range(64) for z in range(64)], np.int32)
+    nb = V.grid_neighbours(64, 64)
+    b = V.Batch(ctx, coords, 32, ctx.terrain(C2), nb, formats=V.FMT_VERTICES)
+    b.set_mode(mode)
+    b.run()
+    total = b.wait()
+    t = oterrain(O, C2)
+    otot, ofaces = O.batch_run(t, coords, nb, 32, THREADS)
+    got = np.array([b.chunk_faces(i)[1] for i in range(len(coords))])
+    firsts = np.array([b.chunk_faces(i)[0] for i in range(len(coords))])
+    assert total == otot == int(got.sum())
+    assert (got == ofaces).all()
+    assert (firsts == np.concatenate([[0], np.cumsum(got)[:-1]])).all()
+    rng = np.random.default_rng(22)
+    check_sample(O, b, t, coords, [32] * len(coords), nb, rng.choice(len(coords), 16, replace=False),
+                 b.mesh(V.FMT_VERTICES), b.mesh(V.FMT_INDICES))
+
+
+@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED])
+def test_config3_full_ragged(ctx, oracle_mod, mode):
+    """2048 chunks, edges uniform over {16, 32, 64}, classes 30% air / 10%
+    solid / 60% 4-octave heightmap surface (one seeded stream, seed 11), no
+    neighbours: counts of all chunks, byte-exact sample, prefix order.  The
+    15.0 M faces exceed the default arena (4 N^2 per chunk, 14.5 M), so the
+    run also exercises the grow-and-re-emit path after either form."""
+    O = oracle_mod
+    say("config 3 full", mode)
+    rng = np.random.default_rng(11)
+    n = 2048
+    edges = rng.choice([16, 32, 64], size=n)
+    cls = rng.choice(3, size=n, p=[0.3, 0.1, 0.6])
+    terrains = {}
+    tids = []
+    for e, k in zip(edges, cls):
+        key = (int(k), int(e))
+        if key not in terrains:
+            if k == 0:
+                tv = V.Terrain(kind=V.TERRAIN_AIR)
+            elif k == 1:
+                tv = V.Terrain(kind=V.TERRAIN_SOLID)
+            else:
+                tv = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=11, octaves=4, frequency=1 / 32,
+                               height_scale=float(e), height_max=int(e) - 1)
+            terrains[key] = (tv, ctx.terrain(tv))
+        tids.append(terrains[key][1])
+    coords = np.array([(i % 64, 0, i // 64) for i in range(n)], np.int32)
+    nb = np.full((n, 4), -1, np.int32)
+    b = V.Batch(ctx, coords, edges, tids, nb, formats=V.FMT_VERTICES)
+    b.set_mode(mode)
+    b.run()
+    total = b.wait()
+    assert total == 15027984
+    got = np.array([b.chunk_faces(i)[1] for i in range(n)])
+    firsts = np.array([b.chunk_faces(i)[0] for i in range(n)])
+    assert total == int(got.sum())
+    assert (firsts == np.concatenate([[0], np.cumsum(got)[:-1]])).all()
+    # per-class closed forms: air 0 faces, solid without neighbours 6 N^2
+    for i in range(n):
+        if cls[i] == 0:
+            assert got[i] == 0
+        elif cls[i] == 1:
+            assert got[i] == 6 * edges[i] ** 2
+    # every surface chunk's count against the oracle, 48 of them byte for byte
+    verts, idxs = b.mesh(V.FMT_VERTICES), b.mesh(V.FMT_INDICES)
+    surf = np.nonzero(cls == 2)[0]
+    sample = set(rng.choice(surf, 48, replace=False).tolist())
+    for i in surf:
+        t = oterrain(O, terrains[(2, int(edges[i]))][0])
+        e = int(edges[i])
+        ids = O.chunk_fill(t, *map(int, coords[i]), e)
+        if i in sample:
+            faces, _, vert, ix = O.mesh_buffers(ids, e, tuple(int(v) * e for v in coords[i]))
+            assert (bits(verts[4 * firsts[i]:4 * (firsts[i] + got[i])]) == bits(vert)).all(), i
+            assert (idxs[6 * firsts[i]:6 * (firsts[i] + got[i])] == ix).all(), i
+        else:
+            faces = O.mesh_faces(ids, e, tuple(int(v) * e for v in coords[i]))
+        assert got[i] == len(faces), i
+
+
+def test_config5_full_resident_grid_and_cull(ctx, oracle_mod):
+    """65,536 resident chunks (256 x 256, seed 5): face counts of all chunks
+    against the oracle, and the cull mask of every AABB for three cameras."""
+    import test_oracle as T
+    O = oracle_mod
+    say("config 5 full")
+    side = 256
+    coords = np.array([(x, 0, z) for x in range(side) for z in range(side)], np.int32)
+    nb = V.grid_neighbours(side, side)
+    b = V.Batch(ctx, coords, 32, ctx.terrain(C5), nb, formats=V.FMT_VERTICES,
+                face_capacity=200_000_000)
+    b.run()
+    total = b.wait()
+    otot, ofaces = O.batch_run(oterrain(O, C5), coords, nb, 32, THREADS)
+    got = np.array([b.chunk_faces(i)[1] for i in range(len(coords))])
+    assert total == otot and (got == ofaces).all()
+    cb = V.DeviceBuffer.of(ctx, coords)
+    mb = V.DeviceBuffer(ctx, len(coords))
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        eye = np.array([rng.uniform(0, 8192), 60.0, rng.uniform(0, 8192)])
+        d = np.array([np.cos(rng.uniform(0, 6.3)), -0.2, np.sin(rng.uniform(0, 6.3))])
+        f = V.Frustum(T.colmajor(T.perspective(np.radians(70), 16 / 9, 0.1, 1024) @
+                                 T.look_at(eye, eye + d)))
+        ctx.check(ctx.L.vx_frustum_mask_device(ctx.h, V.C.byref(f.c), cb.ptr, len(coords), 32,
+                                               mb.ptr), "vx_frustum_mask_device")
+        got_m = mb.download(np.uint8, len(coords))
+        ref_m = O.frustum_contains(f.planes, coords, 32)
+        assert (got_m == ref_m).all() and got_m.sum() > 0
+    cb.free()
+    mb.free()

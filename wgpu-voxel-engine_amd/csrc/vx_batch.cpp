@@ -400,6 +400,14 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
         VX_HIP(ctx, hipMemcpyAsync(b->fused_dev, fu.data(), fu.size() * sizeof(FusedWork),
                                    hipMemcpyHostToDevice, ctx->stream));
     }
+    {
+        // the scan rewrites it every run; start from a valid order regardless
+        std::vector<int32_t> iota_order(n);
+        std::iota(iota_order.begin(), iota_order.end(), 0);
+        VX_HIP(ctx, hipMemcpyAsync(b->emit_order, iota_order.data(), (size_t)n * 4,
+                                   hipMemcpyHostToDevice, ctx->stream));
+        VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // pageable source
+    }
     VX_HIP(ctx, hipMemcpyAsync(b->fill_dev, fw.data(), fw.size() * sizeof(FillWork),
                                hipMemcpyHostToDevice, ctx->stream));
     VX_HIP(ctx, hipMemcpyAsync(b->mesh_dev, mw.data(), mw.size() * sizeof(MeshWork),
@@ -524,12 +532,14 @@ extern "C" int vx_batch_wait(vx_ctx *ctx, vx_batch *b, uint64_t *total_faces) {
         return VX_ERR_DEVICE;
     }
     if (overflow || total > b->capacity) {
-        // grow the arena to the exact total and re-run only the emit stage;
-        // counts and bases from this run stay valid
+        // grow the arena to the exact total and re-run only the meshing
+        // output: the scan again (same bases from this run's counts; it also
+        // writes the emission order, which a fused run never wrote, and clears
+        // the overflow word), then the staged emit
         int st = alloc_arena(b, total + total / 16 + 1024);
         if (st) return st;
         drop_graph(b);  // arena pointers are baked into the captured graph
-        VX_HIP(ctx, hipMemsetAsync(b->overflow, 0, 4, ctx->stream));
+        VX_HIP(ctx, enqueue_scan(b));
         VX_HIP(ctx, enqueue_emit(b));
         VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 4, hipMemcpyDeviceToHost,
                                    ctx->stream));
