@@ -252,7 +252,8 @@ int vx_batch_set_cull(vx_batch *batch, const vx_frustum *f, const int32_t *coord
                       int32_t edge, uint8_t *mask_dev);
 int vx_batch_destroy(vx_batch *batch);
 /* Execution form of a batch:
- *   VX_BATCH_AUTO    fused when eligible, else staged (default)
+ *   VX_BATCH_AUTO    staged, or the form vx_batch_autotune chose for batches
+ *                    of this shape (default)
  *   VX_BATCH_FUSED   one persistent kernel: noise -> fill -> cull -> compact per
  *                    chunk, neighbour planes recomputed, output offsets from a
  *                    single-pass device-wide scan.  Eligible when every in-batch
