@@ -72,7 +72,7 @@ def cpu_baseline(target_s: float = 12.0):
     t0 = time.perf_counter()
     O.batch_run(t, coords, V.grid_neighbours(side, side), EDGE, threads)
     per_chunk = (time.perf_counter() - t0) / len(coords)
-    reps = 3
+    reps = 5
     n_target = max(64, int(target_s / reps / max(per_chunk, 1e-6)))
     side = int(min(GRID, max(8, int(np.sqrt(n_target)))))
     coords = np.array([(x, 0, z) for x in range(side) for z in range(side)], np.int32)

@@ -49,6 +49,11 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* Everything declared here is the library's exported surface; it is built
+ * with hidden visibility otherwise. */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
 
 #define VX_ABI_VERSION 1
 
@@ -460,6 +465,9 @@ int vx_world_list(vx_world *world, uint32_t what, vx_coord *out, int32_t cap, in
  * absent; pass NULL for the one you do not want). */
 int vx_world_lookup(vx_world *world, vx_coord c, vx_chunk **chunk, vx_mesh **mesh);
 
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
 #ifdef __cplusplus
 }
 #endif
