@@ -277,7 +277,12 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
         const int y0 = (item - g * NSEG) * SEG;
         const int li = g * L + lane / N;
         int wxi, wzi;
-        line_column<N>(li, lane, H, ox, oz, wxi, wzi);
+        // Column terms are computed per item, not hoisted: at edge 32 a lane's
+        // x is the same for every item, and hoisted x-lattice terms would be
+        // kept in spilled registers.
+        int lane_i = lane;
+        asm volatile("" : "+v"(lane_i));
+        line_column<N>(li, lane_i, H, ox, oz, wxi, wzi);
         float u[OCT], wz[OCT], c[OCT][8], gy[OCT][8];
         int pX0[OCT], pX1[OCT], Zi[OCT];
 #pragma unroll
@@ -318,15 +323,30 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
                 const int h[8] = {S.perm[AA],     S.perm[BA],     S.perm[AB],
                                   S.perm[BB],     S.perm[AA + 1], S.perm[BA + 1],
                                   S.perm[AB + 1], S.perm[BB + 1]};
+                float cn[8], gn[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
                     const float4 Lk = S.lut[k][h[k] & 15];
                     const float xr = (k & 1) ? fx1 : fx;
                     const float zr = (k & 4) ? fz1 : fz;
-                    const float ck = fmaf(Lk.x, xr, Lk.y * zr);
+                    cn[k] = fmaf(Lk.x, xr, Lk.y * zr);
+                    gn[k] = Lk.z;
+                }
+                // Store into octave o's registers through a uniform branch per
+                // octave with plain moves.  Written as selects, every
+                // coefficient of every octave costs a v_cndmask per rehash
+                // (64 at four octaves, each reading its mask from SGPRs at
+                // half issue rate); the asm keeps the branches from being
+                // turned back into selects.
 #pragma unroll
-                    for (int q = 0; q < OCT; q++)
-                        if (o == q) c[q][k] = ck, gy[q][k] = Lk.z;
+                for (int q = 0; q < OCT; q++) {
+                    if (o == q) {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            asm volatile("v_mov_b32 %0, %1" : "=v"(c[q][k]) : "v"(cn[k]));
+                            asm volatile("v_mov_b32 %0, %1" : "=v"(gy[q][k]) : "v"(gn[k]));
+                        }
+                    }
                 }
             }
             float sum = 0.0f;
@@ -563,11 +583,13 @@ __device__ __forceinline__ void fill_item(const FillWork &w, const TerrainDev *t
     }
 }
 
+// Four waves per SIMD (at most 128 VGPRs) for edges up to 32: left to
+// itself the allocator takes one register more and drops to three waves,
+// about 15% slower.  Edge-64 batches stay at three waves.
 template <int MAXN>
-__global__ void __launch_bounds__(kBlock) fill_kernel(const FillWork *__restrict__ work,
-                                                      const int32_t *__restrict__ order,
-                                                      const TerrainDev *__restrict__ terrains,
-                                                      const GradLut *__restrict__ lut) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+fill_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order,
+            const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut) {
     __shared__ FillShared<MAXN> S;
     const int item = order ? order[blockIdx.x] : (int)blockIdx.x;
     fill_item<MAXN>(work[item], terrains, lut, S);
