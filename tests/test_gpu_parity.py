@@ -18,6 +18,17 @@ C2 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacu
 C5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64, y_scale=1 / 128)
 REF_NOISE = V.Terrain(kind=V.TERRAIN_DENSITY, seed=1, octaves=3, frequency=0.09, lacunarity=0.5,
                       gain=2.5, y_scale=0.0)  # the reference's own fbm parameters, threshold 0
+# octave counts beyond the configs' 1, 3 and 4: 2 runs the octave-inner row
+# loop, 5 and 8 (the maximum) the octave-outer loop; non-power-of-two gain
+# and lacunarity
+OCT2 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=23, octaves=2, frequency=1 / 40, lacunarity=1.9,
+                 gain=0.6, y_scale=1 / 96)
+OCT5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=31, octaves=5, frequency=1 / 100, lacunarity=2.0,
+                 gain=0.5, y_scale=1 / 128)
+OCT8 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=47, octaves=8, frequency=1 / 300, lacunarity=2.1,
+                 gain=0.45, y_scale=1 / 90)
+HMAP8 = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=5, octaves=8, frequency=1 / 50, lacunarity=1.8,
+                  gain=0.55, height_scale=32.0, height_max=31)
 
 
 def say(*a):
@@ -56,6 +67,8 @@ def assert_mesh_equal(O, mesh, ids, edge, coord, adj_ids):
     (C5, (100, 0, -120), 32), (REF_NOISE, (3, 0, -4), 32), (C1, (3, 0, 1), 16),
     (V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=11, octaves=4, frequency=1 / 32,
                height_scale=64.0, height_max=63), (-2, 0, 6), 64),
+    (OCT2, (1, 0, -1), 32), (OCT2, (4, -1, 2), 16), (OCT5, (-6, 0, 3), 32), (OCT5, (0, 0, 1), 64),
+    (OCT8, (2, 0, 2), 32), (OCT8, (-1, 1, -9), 16), (HMAP8, (7, 0, -3), 32),
 ])
 def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
     say("noise", terrain.kind, terrain.seed, coord, edge)
@@ -70,6 +83,7 @@ def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
     (C2, (5, 0, 9), 32), (C2, (63, 0, 63), 32), (C2, (-7, -1, 3), 32), (C2, (4, 0, 4), 16),
     (C2, (1, 0, 1), 64), (REF_NOISE, (0, 0, 0), 32),
     (V.Terrain(kind=V.TERRAIN_AIR), (0, 0, 0), 32), (V.Terrain(kind=V.TERRAIN_SOLID), (2, 0, 1), 64),
+    (OCT2, (3, 0, 3), 32), (OCT5, (1, 0, -2), 32), (OCT8, (0, 0, 0), 64), (HMAP8, (-2, 0, 4), 16),
 ])
 def test_fill_ids_and_occupancy(ctx, oracle_mod, terrain, coord, edge):
     say("fill", terrain.kind, terrain.seed, coord, edge)
@@ -231,6 +245,30 @@ def test_batch_density_grid_other_edges(ctx, oracle_mod, mode, edge):
     coords = [(x - 2, -1 if edge == 16 else 0, z - 1) for x in range(3) for z in range(3)]
     batch_vs_oracle(ctx, oracle_mod, coords, [edge] * 9, [C5] * 9, V.grid_neighbours(3, 3),
                     mode=mode)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("terrain", [pytest.param(OCT2, id="oct2"), pytest.param(OCT5, id="oct5"),
+                                     pytest.param(OCT8, id="oct8"), pytest.param(HMAP8, id="hmap8")])
+def test_batch_octave_counts(ctx, oracle_mod, mode, terrain):
+    """3x3 grids of 32^3 chunks with 2, 5 and 8 octaves (density) and 8
+    octaves (heightmap): fill, halo recompute and meshes byte for byte."""
+    say("batch octaves", terrain.octaves, terrain.kind, mode)
+    coords = [(x + 1, 0, z - 2) for x in range(3) for z in range(3)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [32] * 9, [terrain] * 9, V.grid_neighbours(3, 3),
+                    mode=mode)
+
+
+def test_terrain_validation(ctx):
+    """Octave counts outside 1..8 and octave frequencies below 2^-20 are
+    refused with VX_ERR_INVALID."""
+    import dataclasses
+    for bad in (dataclasses.replace(OCT2, octaves=0), dataclasses.replace(OCT2, octaves=9),
+                dataclasses.replace(OCT8, frequency=1e-6, lacunarity=0.5)):
+        with pytest.raises(V.VoxelError) as e:
+            ctx.terrain(bad)
+        assert e.value.status == V.ERR_INVALID
+    assert ctx.terrain(dataclasses.replace(OCT2, octaves=8)) >= 0
 
 
 def test_fused_is_refused_for_non_adjacent_neighbours(ctx, oracle_mod):
