@@ -624,7 +624,7 @@ constexpr int kStageWords = 32 * 20;  // 32 records per wave pass: 2.5 KB
 template <int MAXN> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
-    uint32_t desc[kDescTile / 2];  // 34 KB in all: 4 workgroups per CU
+    uint32_t desc[kDescTile / 2];  // 24 KB in all: 6 workgroups per CU
     float P[6][2][4][3];
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
