@@ -39,6 +39,34 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert L.vx_frustum_from_matrix(None, None) == V.ERR_INVALID
 
 
+def null_args(argtypes):
+    """Zero / NULL / default-structure arguments for a binding signature."""
+    out = []
+    for a in argtypes:
+        if isinstance(a, type) and issubclass(a, C.Structure):
+            out.append(a())
+        elif a in (C.c_int, C.c_int32, C.c_uint32, C.c_uint64, C.c_int64, C.c_float, C.c_double):
+            out.append(0)
+        else:
+            out.append(None)
+    return out
+
+
+def test_every_entry_point_rejects_null_handles():
+    """No call panics across the ABI: each entry point given NULL handles and
+    pointers (and zero sizes) returns VX_ERR_INVALID, with no device."""
+    L = V.lib()
+    skip = {"vx_abi_version", "vx_status_string", "vx_ctx_last_error"}
+    got = {}
+    for name, (_, argtypes) in sorted(V._SIGS.items()):
+        if name in skip:
+            continue
+        got[name] = getattr(L, name)(*null_args(argtypes))
+    bad = {n: st for n, st in got.items() if st != V.ERR_INVALID}
+    assert bad == {} and len(got) >= 60
+    assert L.vx_ctx_last_error(None) == b"no context"
+
+
 def test_frustum_planes_on_host_match_oracle(oracle_mod):
     rng = np.random.default_rng(0)
     for _ in range(20):
