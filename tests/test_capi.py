@@ -1,6 +1,9 @@
 """The C-ABI library loads and exports every entry point include/voxel.h
 declares; host-only behaviour that needs no GPU.  CPU only."""
 import ctypes as C
+import os
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -84,3 +87,51 @@ def test_no_device_reports_nodev_here():
     with pytest.raises(V.VoxelError) as e:
         V.Context()
     assert e.value.status == V.ERR_NODEV
+
+
+# ABI struct -> its ctypes mirror in the binding (layout must agree)
+MIRRORS = {"vx_coord": "Coord", "vx_terrain": "TerrainC", "vx_ctx_desc": "CtxDesc",
+           "vx_frustum": "FrustumC", "vx_request": "Request", "vx_completion": "Completion",
+           "vx_server_desc": "ServerDesc", "vx_server_stats": "ServerStats",
+           "vx_sim_desc": "SimDesc", "vx_sim_result": "SimResult",
+           "vx_tune_result": "TuneResult", "vx_world_desc": "WorldDesc",
+           "vx_world_stats": "WorldStats", "vx_batch_req": "BatchReq"}
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include "voxel.h"
+#define SZ(t) printf(#t " %zu %zu\n", sizeof(t), _Alignof(t));
+int main(void) {
+    if (vx_abi_version() != VX_ABI_VERSION) return 2;
+    if (vx_chunk_new(NULL, 0, (vx_coord){0, 0, 0}, 32, NULL) != VX_ERR_INVALID) return 3;
+    SZ(vx_coord) SZ(vx_terrain) SZ(vx_ctx_desc) SZ(vx_frustum) SZ(vx_request)
+    SZ(vx_completion) SZ(vx_server_desc) SZ(vx_server_stats) SZ(vx_sim_desc)
+    SZ(vx_sim_result) SZ(vx_tune_result) SZ(vx_world_desc) SZ(vx_world_stats)
+    SZ(vx_batch_req) SZ(vx_draw_indexed)
+    return 0;
+}
+"""
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """A C11 program (what a Rust bindgen or cgo consumer sees) compiles
+    against include/voxel.h with -Wall -Wextra -pedantic -Werror, links the
+    library, calls into it, and sees the same struct sizes and alignments as
+    the ctypes binding."""
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "abi.c"
+    src.write_text(C_PROGRAM)
+    exe = tmp_path / "abi"
+    libdir = os.path.dirname(V.LIB_PATH)
+    inc = os.path.join(os.path.dirname(libdir), "include")
+    subprocess.run([cc, "-std=c11", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", inc,
+                    str(src), "-o", str(exe), "-L", libdir, "-lvoxel_hip",
+                    "-Wl,-rpath," + libdir], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    sizes = {ln.split()[0]: (int(ln.split()[1]), int(ln.split()[2])) for ln in out.splitlines()}
+    for cname, pyname in MIRRORS.items():
+        st = getattr(V, pyname)
+        assert sizes[cname] == (C.sizeof(st), C.alignment(st)), cname
+    assert sizes["vx_draw_indexed"][0] == V.DRAW_DTYPE.itemsize == 20
