@@ -227,6 +227,52 @@ def world_leg(ticks: int = 130):
             "unloaded": st["unloaded"], "resident_chunks": st["chunks"]}
 
 
+def configs_leg(ctx, reps: int = 20):
+    """The other batch configurations, each one captured graph replayed
+    `reps` times after a sizing run: config 1 (4 x 4 chunks of 16^3, 2-D
+    heightmap seed 42, 1 octave, f 1/32, heights 0..15, in-grid neighbours)
+    and config 3 (2048 chunks, edges uniform over 16/32/64, classes 30% air /
+    10% solid / 60% 4-octave heightmap surface with height = edge, one
+    seeded stream, seed 11, no neighbours)."""
+    out = {}
+    c1 = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=42, octaves=1, frequency=1 / 32,
+                   height_scale=16.0, height_max=15)
+    coords1 = [(x, 0, z) for x in range(4) for z in range(4)]
+    b1 = V.Batch(ctx, coords1, 16, ctx.terrain(c1), V.grid_neighbours(4, 4),
+                 formats=V.FMT_VERTICES)
+    rng = np.random.default_rng(11)
+    n3 = 2048
+    edges = rng.choice([16, 32, 64], size=n3)
+    cls = rng.choice(3, size=n3, p=[0.3, 0.1, 0.6])
+    tids = {}
+    for e, k in zip(edges, cls):
+        key = (int(k), int(e))
+        if key not in tids:
+            tv = (V.Terrain(kind=V.TERRAIN_AIR) if k == 0 else V.Terrain(kind=V.TERRAIN_SOLID)
+                  if k == 1 else V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=11, octaves=4,
+                                           frequency=1 / 32, height_scale=float(e),
+                                           height_max=int(e) - 1))
+            tids[key] = ctx.terrain(tv)
+    coords3 = [(i % 64, 0, i // 64) for i in range(n3)]
+    b3 = V.Batch(ctx, coords3, edges, [tids[(int(k), int(e))] for e, k in zip(edges, cls)],
+                 np.full((n3, 4), -1, np.int32), formats=V.FMT_VERTICES)
+    for name, b, n in (("config1", b1, len(coords1)), ("config3", b3, n3)):
+        b.run()
+        faces = b.wait()  # sizes the arena (config 3 outgrows the default once)
+        b.run()
+        b.wait()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"chunks": n, "faces": faces, "ms_per_batch": round(ms, 4),
+                     "chunks_per_s": round(n / (ms * 1e-3), 1)}
+        b.close()
+    return out
+
+
 def measured_counter(kernel: str, faces: int, key: str = "bytes"):
     """A per-launch figure of `kernel` from the committed rocprofv3 summary
     (key "bytes": HBM bytes from FETCH_SIZE / WRITE_SIZE; "valu_insts":
@@ -432,6 +478,10 @@ def main():
         log(f"[bench] frame: {frame['frame_ms']} ms ({frame['form']}), resident generation "
             f"{frame['resident_chunks_per_s']:.0f} chunks/s")
 
+    configs = configs_leg(ctx)
+    log("[bench] other configs: " + ", ".join(
+        f"{k} {v['ms_per_batch']} ms ({v['chunks_per_s']:.0f} chunks/s)" for k, v in configs.items()))
+
     serving = None
     if not args.no_serve:
         serving = serving_leg()
@@ -473,6 +523,7 @@ def main():
         "plain_gpu": plain,
         "other_form_gpu": other,
         "full_frame": frame,
+        "other_configs": configs,
         "serving": serving,
         "world": world,
     }

@@ -624,7 +624,11 @@ constexpr int kStageWords = 32 * 20;  // 32 records per wave pass: 2.5 KB
 template <int MAXN> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
-    uint32_t desc[kDescTile / 2];  // 24 KB in all: 6 workgroups per CU
+    uint32_t desc[kDescTile / 2];
+    // first face of every row, for the interleaved descriptor walk (4 KB at
+    // MAXN = 32: 28 KB in all, 5 workgroups per CU; 16 KB at MAXN = 64,
+    // whose 32 KB of u64 rows already hold it to 2)
+    uint32_t rowfirst[MAXN >= 32 ? MAXN * MAXN : 1];
     float P[6][2][4][3];
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
@@ -804,12 +808,40 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     for (int i = b; i < e; i++) order_out[atomicAdd(&s_hist[order_bucket(counts[i])], 1u)] = i;
 }
 
+// Descriptors (row | x << 12 | dir << 18) of row r's faces that fall in the
+// tile [t0, t1), the row's first face being f, written to S.desc[face - t0]
+// in (x, direction) order.
+template <int N, int MAXN, class SH>
+__device__ __forceinline__ void walk_row(SH &S, int r, uint32_t f, uint32_t t0, uint32_t t1) {
+    using R = typename RowType<MAXN>::T;
+    R m[6];
+    face_masks<N, MAXN>(S, r / N, r % N, m);
+    R any = m[0] | m[1] | m[2] | m[3] | m[4] | m[5];
+    uint32_t g = f;
+    while (any && g < t1) {
+        const int x = ctz(any);
+        any &= any - 1;
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+            if ((m[d] >> x) & 1) {
+                if (g >= t0 && g < t1)
+                    S.desc[g - t0] = (uint32_t)r | ((uint32_t)x << 12) | ((uint32_t)d << 18);
+                g++;
+            }
+        }
+    }
+}
+
 // Emits one chunk's faces.  Each thread owns RPT consecutive rows; a block
 // scan of the per-thread counts gives every row its first face index, so the
 // face order is (y, z, x, direction) exactly as chunk.rs:65-240 pushes them.
-// Faces are staged as 32-bit descriptors (row | x << 12 | dir << 18) in LDS
-// tiles (the size of S.desc), then one lane per face writes its records.
-template <int N, int MAXN, class SH>
+// Faces are staged as 32-bit descriptors in LDS tiles (the size of S.desc),
+// then one lane per face writes its records.  INTERLEAVE (the emit kernels
+// for MAXN >= 32, any edge): the rows' first faces go to S.rowfirst and the
+// descriptor walk deals rows to threads round-robin, since a tile's faces
+// often come from a band of consecutive rows (a surface) that consecutive
+// ownership (4 or 16 rows per thread) would leave to a handful of threads.
+template <int N, int MAXN, bool INTERLEAVE, class SH>
 __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
                           SH &S) {
     using R = typename RowType<MAXN>::T;
@@ -849,35 +881,34 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
     uint32_t total;
     const uint32_t first = block_exclusive_scan(mine, S.red, &total);
     (void)total;  // equals F (count kernel); F is authoritative for sizing
-
+    if constexpr (INTERLEAVE) {
+        uint32_t f = first;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            S.rowfirst[r0 + i] = f;
+            f += rc[i];
+        }
+        __syncthreads();
+    }
 
     constexpr uint32_t TILE = sizeof(S.desc) / sizeof(S.desc[0]);
     for (uint32_t t0 = 0; t0 < F; t0 += TILE) {
         const uint32_t t1 = min(F, t0 + TILE);
-        uint32_t f = first;
-        if (f < t1 && f + mine > t0) {
+        if constexpr (INTERLEAVE) {
 #pragma unroll 1
-            for (int i = 0; i < RPT; i++) {
-                const int r = r0 + i;
-                if (f + rc[i] > t0 && f < t1) {
-                    R m[6];
-                    face_masks<N, MAXN>(S, r / N, r % N, m);
-                    R any = m[0] | m[1] | m[2] | m[3] | m[4] | m[5];
-                    uint32_t g = f;
-                    while (any && g < t1) {
-                        const int x = ctz(any);
-                        any &= any - 1;
-#pragma unroll
-                        for (int d = 0; d < 6; d++) {
-                            if ((m[d] >> x) & 1) {
-                                if (g >= t0 && g < t1)
-                                    S.desc[g - t0] = (uint32_t)r | ((uint32_t)x << 12) | ((uint32_t)d << 18);
-                                g++;
-                            }
-                        }
-                    }
+            for (int r = tid; r < N * N; r += kBlock) {
+                const uint32_t f = S.rowfirst[r];
+                const uint32_t e = r + 1 < N * N ? S.rowfirst[r + 1] : F;
+                if (e > t0 && f < t1) walk_row<N, MAXN>(S, r, f, t0, t1);
+            }
+        } else {
+            uint32_t f = first;
+            if (f < t1 && f + mine > t0) {
+#pragma unroll 1
+                for (int i = 0; i < RPT; i++) {
+                    if (f + rc[i] > t0 && f < t1) walk_row<N, MAXN>(S, r0 + i, f, t0, t1);
+                    f += rc[i];
                 }
-                f += rc[i];
             }
         }
         __syncthreads();
@@ -972,7 +1003,7 @@ __device__ void emit_chunk(const MeshWork &w, uint64_t base, uint32_t F, const M
                            MeshShared<MAXN> &S) {
     load_rows<N, MAXN>(w, S);
     __syncthreads();
-    emit_rows<N, MAXN>(w, base, F, out, S);
+    emit_rows<N, MAXN, (MAXN >= 32)>(w, base, F, out, S);
 }
 
 template <int MAXN>
@@ -994,11 +1025,11 @@ __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint
     }
 }
 
-// Register cap: six waves per SIMD (80 VGPRs; the 24 KB of LDS also allows
-// six workgroups per CU), so more workgroups overlap their descriptor and
-// store phases.
+// Register cap: five waves per SIMD (96 VGPRs), matching the five
+// workgroups per CU that the 28 KB of LDS allows at MAXN = 32, so more
+// workgroups overlap their descriptor and store phases.
 template <int MAXN>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) emit_kernel(const MeshWork *__restrict__ work,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) emit_kernel(const MeshWork *__restrict__ work,
                                                       const int32_t *__restrict__ order,
                                                       const uint64_t *__restrict__ bases,
                                                       const uint32_t *__restrict__ counts,
@@ -1137,7 +1168,7 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
     if (base + F > out.capacity) {
         if (tid == 0) atomicOr(ctl.overflow, 1u);
     } else {
-        emit_rows<N, MAXN>(W.m, base, F, out, S);
+        emit_rows<N, MAXN, false>(W.m, base, F, out, S);
     }
     __syncthreads();
 }
