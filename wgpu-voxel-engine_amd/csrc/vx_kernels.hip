@@ -1068,6 +1068,7 @@ template <int MAXN> struct FusedShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t desc[kDescTile];
+    uint32_t rowfirst[MAXN == 32 ? MAXN * MAXN : 1];  // interleaved descriptor walk
     float P[6][2][4][3];
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
@@ -1168,7 +1169,7 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
     if (base + F > out.capacity) {
         if (tid == 0) atomicOr(ctl.overflow, 1u);
     } else {
-        emit_rows<N, MAXN, false>(W.m, base, F, out, S);
+        emit_rows<N, MAXN, (MAXN == 32)>(W.m, base, F, out, S);
     }
     __syncthreads();
 }
