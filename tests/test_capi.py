@@ -2,6 +2,7 @@
 declares; host-only behaviour that needs no GPU.  CPU only."""
 import ctypes as C
 import os
+import re
 import shutil
 import subprocess
 
@@ -113,11 +114,8 @@ int main(void) {
 """
 
 
-def test_header_is_plain_c_and_links(tmp_path):
-    """A C11 program (what a Rust bindgen or cgo consumer sees) compiles
-    against include/voxel.h with -Wall -Wextra -pedantic -Werror, links the
-    library, calls into it, and sees the same struct sizes and alignments as
-    the ctypes binding."""
+def c_layouts(tmp_path):
+    """(size, alignment) of every ABI struct as a C compiler lays it out."""
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
@@ -130,7 +128,72 @@ def test_header_is_plain_c_and_links(tmp_path):
                     str(src), "-o", str(exe), "-L", libdir, "-lvoxel_hip",
                     "-Wl,-rpath," + libdir], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
-    sizes = {ln.split()[0]: (int(ln.split()[1]), int(ln.split()[2])) for ln in out.splitlines()}
+    return {ln.split()[0]: (int(ln.split()[1]), int(ln.split()[2])) for ln in out.splitlines()}
+
+
+def header_arity():
+    """Parameter count of every function the header declares."""
+    text = re.sub(r"/\*.*?\*/", "", open(V.HEADER).read(), flags=re.S)
+    out = {}
+    for name, params in re.findall(r"(vx_[a-z0-9_]+)\s*\(([^()]*(?:\([^()]*\)[^()]*)*)\)\s*;", text):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def rust_stub():
+    text = open(os.path.join(os.path.dirname(os.path.dirname(V.LIB_PATH)), "INTEGRATION.md")).read()
+    return re.search(r"```rust\n(.*?)```", text, re.S).group(1)
+
+
+def rust_layout(ty, structs):
+    """(size, alignment) of a Rust field type under #[repr(C)]."""
+    ty = ty.strip()
+    prim = {"i32": 4, "u32": 4, "f32": 4, "i64": 8, "u64": 8}
+    if ty in prim:
+        return prim[ty], prim[ty]
+    m = re.fullmatch(r"\[(.+);\s*(\d+)\]", ty)
+    if m:
+        sz, al = rust_layout(m.group(1), structs)
+        return sz * int(m.group(2)), al
+    return structs[ty]
+
+
+def test_integration_rust_stub_matches_the_header(tmp_path):
+    """The Rust binding shown in INTEGRATION.md declares only functions the
+    header has, with the same number of parameters, and every #[repr(C)]
+    struct it spells out has the C layout (size and alignment)."""
+    stub = rust_stub()
+    arity = header_arity()
+    fns = re.findall(r"pub fn (vx_[a-z0-9_]+)\((.*?)\)\s*->", stub, re.S)
+    assert len(fns) >= 20
+    for name, params in fns:
+        n = 0 if not params.strip() else params.count(",") + 1 - params.strip().endswith(",")
+        assert name in arity, name
+        assert n == arity[name], (name, n, arity[name])
+    c = c_layouts(tmp_path)
+    structs = {}
+    for name, body in re.findall(r"#\[repr\(C\)\][^\n]*?pub struct (vx_[a-z_]+) \{(.*?)\}", stub, re.S):
+        if "_p: [u8; 0]" in body:
+            continue  # opaque handle
+        off, align = 0, 1
+        for fname, fty in re.findall(r"pub (\w+):\s*([^,]+?)\s*(?:,|$)", body.strip()):
+            sz, al = rust_layout(fty, structs)
+            off = (off + al - 1) // al * al + sz
+            align = max(align, al)
+        structs[name] = ((off + align - 1) // align * align, align)
+        if name in c:
+            assert structs[name] == c[name], (name, structs[name], c[name])
+    assert {"vx_coord", "vx_terrain", "vx_request", "vx_completion", "vx_world_desc",
+            "vx_draw_indexed"} <= set(structs)
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """A C11 program (what a Rust bindgen or cgo consumer sees) compiles
+    against include/voxel.h with -Wall -Wextra -pedantic -Werror, links the
+    library, calls into it, and sees the same struct sizes and alignments as
+    the ctypes binding."""
+    sizes = c_layouts(tmp_path)
     for cname, pyname in MIRRORS.items():
         st = getattr(V, pyname)
         assert sizes[cname] == (C.sizeof(st), C.alignment(st)), cname
