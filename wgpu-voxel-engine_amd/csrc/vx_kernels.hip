@@ -837,7 +837,9 @@ __device__ __forceinline__ void walk_row(SH &S, int r, uint32_t f, uint32_t t0, 
 // face order is (y, z, x, direction) exactly as chunk.rs:65-240 pushes them.
 // Faces are staged as 32-bit descriptors in LDS tiles (the size of S.desc),
 // then one lane per face writes its records.  INTERLEAVE (the emit kernels
-// for MAXN >= 32, any edge): the rows' first faces go to S.rowfirst and the
+// for MAXN >= 32 and the fused kernel for MAXN = 32, whatever the chunk's
+// edge; S.rowfirst holds MAXN^2 entries): the rows' first faces go to
+// S.rowfirst and the
 // descriptor walk deals rows to threads round-robin, since a tile's faces
 // often come from a band of consecutive rows (a surface) that consecutive
 // ownership (4 or 16 rows per thread) would leave to a handful of threads.
