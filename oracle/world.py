@@ -15,14 +15,28 @@ from __future__ import annotations
 import math
 from collections import deque
 
+import numpy as np
+
 import oracle as O
 
 EDGE = 32  # CHUNK_I32 (src/world.rs:21-22)
 
 
+def as_i32(v):
+    """Rust's `f as i32`: truncation toward zero, saturating, NaN -> 0."""
+    if math.isnan(v):
+        return 0
+    if v >= 2.0 ** 31:
+        return 2 ** 31 - 1
+    if v < -2.0 ** 31:
+        return -2 ** 31
+    return int(v)
+
+
 def to_chunk_coord(pos):
-    """Coord3DF::to_chunk_coord (src/coordinate.rs:33-38): floor, as i32, >> 5."""
-    return tuple(int(math.floor(float(v))) >> 5 for v in pos)
+    """Coord3DF::to_chunk_coord (src/coordinate.rs:33-38): floor, as i32, >> 5
+    (positions are f32)."""
+    return tuple(as_i32(float(np.floor(np.float32(v)))) >> 5 for v in pos)
 
 
 class WorldModel:

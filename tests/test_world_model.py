@@ -103,3 +103,12 @@ def test_delay_zero_receives_in_the_same_tick(oracle_mod):
                      result_delay=0)
     m.tick((16.0, 40.0, 16.0), ALL)
     assert m.received_chunks == 1 and m.in_process == []
+
+
+def test_chunk_coordinate_conversion():
+    """to_chunk_coord: floor then Rust's saturating `as i32`, then >> 5
+    (floor division by 32 for in-range values, clamped at the i32 range)."""
+    import world as W
+    assert W.to_chunk_coord((0.0, 31.9, 32.0)) == (0, 0, 1)
+    assert W.to_chunk_coord((-0.5, -32.0, -32.5)) == (-1, -1, -2)
+    assert W.to_chunk_coord((3e9, -3e9, float("nan"))) == ((2 ** 31 - 1) >> 5, -(2 ** 26), 0)

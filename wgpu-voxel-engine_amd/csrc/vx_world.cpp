@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <deque>
 #include <map>
 #include <tuple>
@@ -34,10 +35,18 @@ Key key_of(vx_coord c) { return Key{c.x, c.y, c.z}; }
 bool same(vx_coord a, vx_coord b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 bool less(vx_coord a, vx_coord b) { return key_of(a) < key_of(b); }
 
-// Coord3DF::to_chunk_coord (src/coordinate.rs:33-38): floor, then >> 5.
+// Rust's `f as i32`: truncation, saturating at the i32 range, NaN -> 0.
+int32_t as_i32(float f) {
+    if (std::isnan(f)) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f < -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+
+// Coord3DF::to_chunk_coord (src/coordinate.rs:33-38): floor, as i32, >> 5.
 vx_coord chunk_of(const float p[3]) {
-    return vx_coord{(int32_t)std::floor(p[0]) >> 5, (int32_t)std::floor(p[1]) >> 5,
-                    (int32_t)std::floor(p[2]) >> 5};
+    return vx_coord{as_i32(std::floor(p[0])) >> 5, as_i32(std::floor(p[1])) >> 5,
+                    as_i32(std::floor(p[2])) >> 5};
 }
 
 struct Job {
