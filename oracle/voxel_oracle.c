@@ -57,6 +57,12 @@ void vxo_perm_from_table(const uint8_t table[256], uint8_t perm[512]) {
 /* Improved Noise (K. Perlin, "Improving Noise", SIGGRAPH 2002, and his      */
 /* reference ImprovedNoise.java).  One source text, instantiated for float   */
 /* (the pipeline) and double (to check Perlin's published known answer).     */
+/* Lattice cell mod 256 of a floored coordinate: (int)fl & 255 while the
+ * conversion is in range, else 0 -- every f32 of magnitude >= 2^31 is a
+ * multiple of 256 (and NaN, from an infinite coordinate, maps to 0 too; the
+ * AVX2 path's _mm256_cvttps_epi32 gives 0x80000000 there, also 0 mod 256). */
+#define VXO_CELL8(fl) (fabs((double)(fl)) < 2147483648.0 ? ((int)(fl) & 255) : 0)
+
 /* ------------------------------------------------------------------------ */
 #define VXO_DEFINE_NOISE3(NAME, T, FLOOR)                                                   \
     static inline T NAME##_fade(T t) { return ((t * t) * t) * (t * (t * (T)6 - (T)15) + (T)10); } \
@@ -69,7 +75,7 @@ void vxo_perm_from_table(const uint8_t table[256], uint8_t perm[512]) {
     }                                                                                       \
     static T NAME(const uint8_t *p, T x, T y, T z) {                                        \
         T fx = FLOOR(x), fy = FLOOR(y), fz = FLOOR(z);                                      \
-        int X = (int)fx & 255, Y = (int)fy & 255, Z = (int)fz & 255;                        \
+        int X = VXO_CELL8(fx), Y = VXO_CELL8(fy), Z = VXO_CELL8(fz);                        \
         x = x - fx;                                                                         \
         y = y - fy;                                                                         \
         z = z - fz;                                                                         \
@@ -113,7 +119,7 @@ static inline float grad2f(int hash, float x, float y) {
 
 float vxo_noise2(const uint8_t *p, float x, float y) {
     float fx = floorf(x), fy = floorf(y);
-    int X = (int)fx & 255, Y = (int)fy & 255;
+    int X = VXO_CELL8(fx), Y = VXO_CELL8(fy);
     x = x - fx;
     y = y - fy;
     float u = noise3f_fade(x), v = noise3f_fade(y);

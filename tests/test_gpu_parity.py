@@ -27,6 +27,13 @@ OCT5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=31, octaves=5, frequency=1 / 100, 
                  gain=0.5, y_scale=1 / 128)
 OCT8 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=47, octaves=8, frequency=1 / 300, lacunarity=2.1,
                  gain=0.45, y_scale=1 / 90)
+# octave frequencies 0.3 .. 2.4 near the ends of the i32 world range: x and
+# z lattice coordinates reach 5e9, past the range of an int conversion (their
+# fractions are 0 there, so the noise comes from y and the cells' hashes)
+BIGF = V.Terrain(kind=V.TERRAIN_DENSITY, seed=3, octaves=4, frequency=0.3, lacunarity=2.0,
+                 gain=0.5, y_scale=1 / 128)
+HMAPBIG = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=9, octaves=4, frequency=1.0, lacunarity=2.0,
+                    gain=0.5, height_scale=32.0, height_max=31)
 HMAP8 = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=5, octaves=8, frequency=1 / 50, lacunarity=1.8,
                   gain=0.55, height_scale=32.0, height_max=31)
 
@@ -69,6 +76,8 @@ def assert_mesh_equal(O, mesh, ids, edge, coord, adj_ids):
                height_scale=64.0, height_max=63), (-2, 0, 6), 64),
     (OCT2, (1, 0, -1), 32), (OCT2, (4, -1, 2), 16), (OCT5, (-6, 0, 3), 32), (OCT5, (0, 0, 1), 64),
     (OCT8, (2, 0, 2), 32), (OCT8, (-1, 1, -9), 16), (HMAP8, (7, 0, -3), 32),
+    (BIGF, (2 ** 26 - 1, 0, -(2 ** 26)), 32), (BIGF, (-(2 ** 26), 3, 2 ** 26 - 1), 32),
+    (HMAPBIG, (2 ** 26 - 1, 0, 2 ** 26 - 2), 32),
 ])
 def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
     say("noise", terrain.kind, terrain.seed, coord, edge)

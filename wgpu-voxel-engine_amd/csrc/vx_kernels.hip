@@ -71,9 +71,17 @@ __device__ __forceinline__ float grad2(int hash, float x, float y) {
     return (h & 1) ? -y : y;
 }
 
+// Lattice cell mod 256 of a floored coordinate.  Every f32 of magnitude
+// >= 2^31 is a multiple of 256, so its cell index is 0; the test also keeps
+// the conversion to int in range (and maps NaN, from an infinite coordinate,
+// to 0).
+__device__ __forceinline__ int cell8(float fl) {
+    return fabsf(fl) < 2147483648.0f ? ((int)fl & 255) : 0;
+}
+
 __device__ float noise2(const int *p, float x, float y) {
     const float fx = floorf(x), fy = floorf(y);
-    const int X = (int)fx & 255, Y = (int)fy & 255;
+    const int X = cell8(fx), Y = cell8(fy);
     x = x - fx;
     y = y - fy;
     const float u = fade(x), v = fade(y);
@@ -188,11 +196,11 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
                 const float X = (float)wxi * f;
                 const float fxl = floorf(X);
                 const float fx = X - fxl, fx1 = fx - 1.0f, u = fade(fx);
-                const int Xi = (int)fxl & 255;
+                const int Xi = cell8(fxl);
                 const float Z = (float)wzi * f;
                 const float fzl = floorf(Z);
                 const float fz = Z - fzl, fz1 = fz - 1.0f, wz = fade(fz);
-                const int Zi = (int)fzl & 255;
+                const int Zi = cell8(fzl);
                 const int pX0 = S.perm[Xi], pX1 = S.perm[Xi + 1];
                 float c[8], gy[8];
                 int yprev = -1;
@@ -290,13 +298,13 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
             const float X = (float)wxi * fo[o];
             const float fxl = floorf(X);
             u[o] = fade(X - fxl);
-            const int Xi = (int)fxl & 255;
+            const int Xi = cell8(fxl);
             pX0[o] = S.perm[Xi];
             pX1[o] = S.perm[Xi + 1];
             const float Z = (float)wzi * fo[o];
             const float fzl = floorf(Z);
             wz[o] = fade(Z - fzl);
-            Zi[o] = (int)fzl & 255;
+            Zi[o] = cell8(fzl);
         }
 #pragma unroll 1
         for (int y = y0; y < y0 + SEG; y++) {
@@ -408,7 +416,7 @@ __device__ __forceinline__ void build_ytab(const TerrainDev &T, int oy, SH &S) {
         const float Y = (float)(oy + y) * f;
         const float fyl = floorf(Y);
         const float fy = Y - fyl;
-        S.ytab[i] = make_float4(fy, fy - 1.0f, fade(fy), __int_as_float((int)fyl & 255));
+        S.ytab[i] = make_float4(fy, fy - 1.0f, fade(fy), __int_as_float(cell8(fyl)));
     }
     // Rows where an octave enters a new lattice cell (the hash uses the cell
     // index mod 256, which is what is compared); row 0 rehashes everything.
@@ -416,8 +424,8 @@ __device__ __forceinline__ void build_ytab(const TerrainDev &T, int oy, SH &S) {
         uint32_t m = 0;
         float f = T.frequency;
         for (int o = 0; o < T.octaves; o++) {
-            const int a = (int)floorf((float)(oy + y) * f) & 255;
-            const int b = (int)floorf((float)(oy + y - 1) * f) & 255;
+            const int a = cell8(floorf((float)(oy + y) * f));
+            const int b = cell8(floorf((float)(oy + y - 1) * f));
             if (y == 0 || a != b) m |= 1u << o;
             f = f * T.lacunarity;
         }
