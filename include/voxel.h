@@ -35,7 +35,9 @@
  *     call them.  GPU work is queued on the context's stream.
  *   - Coordinates are chunk coordinates (ChunkCoord3D, src/coordinate.rs:41-66);
  *     a chunk of edge N covers world voxels [c*N, c*N + N) on each axis (the
- *     reference fixes N = 32, src/coordinate.rs:60-65).
+ *     reference fixes N = 32, src/coordinate.rs:60-65).  Calls that build a
+ *     chunk return VX_ERR_INVALID unless c*N - 1 and c*N + N fit in int32 on
+ *     every axis.
  *   - Block ids are u8: 0 air, 1 grass, 2 dirt (the reference's CubeType
  *     GRASS=0, DIRT=1, AIR=2, src/cube.rs:16-21, shifted so 0 is air).
  *   - Voxel index inside a chunk: x + N*z + N*N*y (src/chunk.rs:38).

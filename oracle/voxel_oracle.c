@@ -586,7 +586,9 @@ void vxo_frustum_planes(const float m[16], float planes[24]) {
  * AABB has dist = ((a*x + b*y) + c*z) + d >= 0 (or NaN) for all six planes. */
 int vxo_frustum_contains(const float planes[24], int32_t cx, int32_t cy, int32_t cz,
                          int32_t edge) {
-    const float px = (float)(cx * edge), py = (float)(cy * edge), pz = (float)(cz * edge);
+    /* exact 64-bit products, then f32 (the i32 product where it does not overflow) */
+    const float px = (float)((int64_t)cx * edge), py = (float)((int64_t)cy * edge),
+                pz = (float)((int64_t)cz * edge);
     const float e = (float)edge;
     float cs[8][3] = {{px, py, pz},         {px + e, py, pz},         {px, py, pz + e},
                       {px + e, py, pz + e}, {px, py + e, pz},         {px + e, py + e, pz},

@@ -80,6 +80,12 @@ def test_frustum_planes_on_host_match_oracle(oracle_mod):
         coords = rng.integers(-4, 4, size=(50, 3))
         host = [f.contains(tuple(int(v) for v in c)) for c in coords]
         assert host == list(oracle_mod.frustum_contains(f.planes, coords).astype(bool))
+        # far coordinates: corner positions are exact 64-bit products rounded
+        # to f32, defined even where the 32-bit product would overflow
+        far = np.concatenate([rng.integers(-2 ** 31, 2 ** 31, size=(20, 3)),
+                              [[2 ** 31 - 1, -2 ** 31, 2 ** 26]]]).astype(np.int64)
+        host = [f.contains(tuple(int(v) for v in c)) for c in far]
+        assert host == list(oracle_mod.frustum_contains(f.planes, far.astype(np.int32)).astype(bool))
 
 
 def test_no_device_reports_nodev_here():

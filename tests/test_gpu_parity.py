@@ -76,8 +76,8 @@ def assert_mesh_equal(O, mesh, ids, edge, coord, adj_ids):
                height_scale=64.0, height_max=63), (-2, 0, 6), 64),
     (OCT2, (1, 0, -1), 32), (OCT2, (4, -1, 2), 16), (OCT5, (-6, 0, 3), 32), (OCT5, (0, 0, 1), 64),
     (OCT8, (2, 0, 2), 32), (OCT8, (-1, 1, -9), 16), (HMAP8, (7, 0, -3), 32),
-    (BIGF, (2 ** 26 - 1, 0, -(2 ** 26)), 32), (BIGF, (-(2 ** 26), 3, 2 ** 26 - 1), 32),
-    (HMAPBIG, (2 ** 26 - 1, 0, 2 ** 26 - 2), 32),
+    (BIGF, (2 ** 26 - 2, 0, 1 - 2 ** 26), 32), (BIGF, (1 - 2 ** 26, 3, 2 ** 26 - 2), 32),
+    (HMAPBIG, (2 ** 26 - 2, 0, 2 ** 26 - 2), 32),
 ])
 def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
     say("noise", terrain.kind, terrain.seed, coord, edge)
@@ -266,6 +266,27 @@ def test_batch_octave_counts(ctx, oracle_mod, mode, terrain):
     coords = [(x + 1, 0, z - 2) for x in range(3) for z in range(3)]
     batch_vs_oracle(ctx, oracle_mod, coords, [32] * 9, [terrain] * 9, V.grid_neighbours(3, 3),
                     mode=mode)
+
+
+def test_coordinates_outside_the_i32_world_are_refused(ctx, oracle_mod):
+    """A chunk whose voxels or one-voxel halo would leave the i32 world range
+    is refused (VX_ERR_INVALID) by every entry point that builds chunks; the
+    last accepted chunk on each side fills bit-exact."""
+    tid = ctx.terrain(BIGF)
+    for coord in ((2 ** 26 - 1, 0, 0), (0, -(2 ** 26), 0), (0, 0, 2 ** 30)):
+        with pytest.raises(V.VoxelError) as e:
+            V.Chunk.new(ctx, BIGF, coord, 32)
+        assert e.value.status == V.ERR_INVALID
+        with pytest.raises(V.VoxelError) as e:
+            V.Batch(ctx, [coord], 32, tid, np.full((1, 4), -1, np.int32))
+        assert e.value.status == V.ERR_INVALID
+    with V.Server(ctx) as s:
+        with pytest.raises(V.VoxelError) as e:
+            s.submit([((2 ** 26 - 1, 0, 0), 32, tid, V.NB_NONE)])
+        assert e.value.status == V.ERR_INVALID
+    for coord in ((2 ** 26 - 2, 0, 0), (1 - 2 ** 26, 0, 0)):
+        ch = V.Chunk.new(ctx, BIGF, coord, 32)
+        assert (ch.ids() == oracle_mod.chunk_fill(oterrain(oracle_mod, BIGF), *coord, 32)).all()
 
 
 def test_terrain_validation(ctx):

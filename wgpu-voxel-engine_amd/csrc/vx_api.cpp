@@ -511,7 +511,7 @@ vx_chunk *new_chunk(vx_ctx *ctx, vx_coord coord, int edge) {
 
 extern "C" int vx_chunk_new(vx_ctx *ctx, int32_t terrain_id, vx_coord coord, int32_t edge,
                             vx_chunk **out) {
-    if (!ctx || !out || !valid_edge(edge)) return VX_ERR_INVALID;
+    if (!ctx || !out || !valid_edge(edge) || !coord_in_range(coord, edge)) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (terrain_id < 0 || terrain_id >= (int)ctx->terrains.size()) return VX_ERR_INVALID;
     vx_chunk *c = new_chunk(ctx, coord, edge);
@@ -537,7 +537,8 @@ extern "C" int vx_chunk_new(vx_ctx *ctx, int32_t terrain_id, vx_coord coord, int
 
 extern "C" int vx_chunk_from_ids(vx_ctx *ctx, vx_coord coord, int32_t edge, const uint8_t *ids,
                                  vx_chunk **out) {
-    if (!ctx || !out || !ids || !valid_edge(edge)) return VX_ERR_INVALID;
+    if (!ctx || !out || !ids || !valid_edge(edge) || !coord_in_range(coord, edge))
+        return VX_ERR_INVALID;
     const size_t nv = (size_t)edge * edge * edge;
     for (size_t i = 0; i < nv; i++)
         if (ids[i] > kBlockDirt) return VX_ERR_INVALID;
@@ -628,7 +629,7 @@ extern "C" int vx_chunk_read_occupancy(vx_ctx *ctx, const vx_chunk *c, uint32_t 
 
 extern "C" int vx_chunk_noise(vx_ctx *ctx, int32_t terrain_id, vx_coord coord, int32_t edge,
                               float *dst, uint64_t cap) {
-    if (!ctx || !dst || !valid_edge(edge)) return VX_ERR_INVALID;
+    if (!ctx || !dst || !valid_edge(edge) || !coord_in_range(coord, edge)) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (terrain_id < 0 || terrain_id >= (int)ctx->terrains.size()) return VX_ERR_INVALID;
     const int kind = ctx->terrains[terrain_id].kind;
@@ -829,7 +830,10 @@ extern "C" int vx_frustum_from_matrix(const float m[16], vx_frustum *out) {
 // contains (src/frustum_culling.rs:16-40) for one chunk, on the host.
 extern "C" int vx_frustum_contains(const vx_frustum *f, vx_coord c, int32_t edge) {
     if (!f || edge <= 0) return VX_ERR_INVALID;
-    const float px = (float)(c.x * edge), py = (float)(c.y * edge), pz = (float)(c.z * edge);
+    // corner positions exact in 64 bits, then rounded to f32 (equal to the
+    // reference's i32 product wherever that does not overflow)
+    const float px = (float)((int64_t)c.x * edge), py = (float)((int64_t)c.y * edge),
+                pz = (float)((int64_t)c.z * edge);
     const float e = (float)edge;
     for (int k = 0; k < 8; k++) {
         const float x = (k & 1) ? px + e : px;

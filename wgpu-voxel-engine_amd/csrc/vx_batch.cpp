@@ -274,6 +274,7 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
     for (int i = 0; i < n; i++) {
         const vx_batch_req &r = reqs[i];
         if (r.edge != 16 && r.edge != 32 && r.edge != 64) return VX_ERR_INVALID;
+        if (!coord_in_range(r.coord, r.edge)) return VX_ERR_INVALID;
         if (r.terrain_id < 0 || r.terrain_id >= (int)ctx->terrains.size()) return VX_ERR_INVALID;
         for (int k = 0; k < 4; k++) {
             const int j = r.nb[k];

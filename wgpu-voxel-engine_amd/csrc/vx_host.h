@@ -44,6 +44,20 @@ struct SharedBlock {
     void *ptr = nullptr;
 };
 
+// A chunk at `c` with this edge is accepted when its voxels and the one-voxel
+// halo around them (the neighbour planes the fused kernel recomputes) have
+// i32 world coordinates: c * edge - 1 >= INT32_MIN and c * edge + edge <=
+// INT32_MAX on every axis.  The kernels then do all world-coordinate
+// arithmetic in 32 bits without overflow.
+inline bool coord_in_range(vx_coord c, int32_t edge) {
+    const int64_t v[3] = {c.x, c.y, c.z};
+    for (int64_t a : v) {
+        const int64_t o = a * edge;
+        if (o - 1 < INT32_MIN || o + edge > INT32_MAX) return false;
+    }
+    return true;
+}
+
 }  // namespace vx
 
 struct vx_ctx {

@@ -1188,7 +1188,10 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
 // workgroup culls a grid-stride slice before it takes chunk tickets
 // (frustum_culling.rs:16-40; dist = ((a*x + b*y) + c*z) + d, NaN passes).
 __device__ __forceinline__ uint8_t aabb_visible(const float *pl, int cx, int cy, int cz, int edge) {
-    const float px = (float)(cx * edge), py = (float)(cy * edge), pz = (float)(cz * edge);
+    // exact 64-bit products rounded to f32: equal to the 32-bit product
+    // wherever that does not overflow, defined for every coordinate
+    const float px = (float)((long long)cx * edge), py = (float)((long long)cy * edge),
+                pz = (float)((long long)cz * edge);
     const float e = (float)edge;
     const float xs[2] = {px, px + e}, ys[2] = {py, py + e}, zs[2] = {pz, pz + e};
     uint32_t any = 0;

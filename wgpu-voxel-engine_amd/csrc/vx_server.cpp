@@ -594,7 +594,8 @@ extern "C" int vx_server_submit(vx_server *S, const vx_request *reqs, int32_t n,
     for (int i = 0; i < n; i++) {
         const vx_request &r = reqs[i];
         if ((r.edge != 16 && r.edge != 32 && r.edge != 64) || r.terrain_id < 0 ||
-            r.terrain_id >= (int)S->ctx->terrains.size() || r.neighbours > VX_NB_RESIDENT)
+            r.terrain_id >= (int)S->ctx->terrains.size() || r.neighbours > VX_NB_RESIDENT ||
+            !coord_in_range(r.coord, r.edge))
             return VX_ERR_INVALID;
     }
     const uint64_t t = now_ns();
