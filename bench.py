@@ -498,7 +498,8 @@ def main():
         log(f"[bench] CPU: {cpu['value']} chunks/s on {cpu['cores']} threads")
 
     out = {
-        "metric": "chunks/sec (noise+mesh, 32^3, bit-exact)",
+        "metric": "chunks/sec (noise+mesh, 32\u00b3, bit-exact) vs CPU & plain GPU; "
+                  "p50/p99 chunk latency",
         "value": round(value, 1),
         "unit": "chunks/s",
         "n_gpus": 1,
