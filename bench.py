@@ -528,6 +528,12 @@ def main():
         "serving": serving,
         "world": world,
     }
+    if serving:
+        # request-to-mesh latency of the streaming config at its highest rate
+        top = max(serving, key=lambda r: r["rate_req_s"])
+        out["latency_ms"] = {"rate_req_s": top["rate_req_s"], "p50": top["p50_ms"],
+                             "p99": top["p99_ms"], "miss_p50": top["miss_p50_ms"],
+                             "miss_p99": top["miss_p99_ms"]}
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)
     if plain:
