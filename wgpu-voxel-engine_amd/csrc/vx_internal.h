@@ -11,6 +11,7 @@ namespace vx {
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kMaxOctaves = 8;     // fBm octaves supported by the noise tables
 constexpr int kDescTile = 4096;    // face descriptors staged in LDS per emit tile
+constexpr int kEmitGrid = 1280;    // emit workgroups that fill the GPU (5 per CU)
 
 // Terrain kinds (mirror of vx_terrain.kind in include/voxel.h).
 enum : int32_t { kAir = 0, kSolid = 1, kHeightmap = 2, kDensity = 3 };

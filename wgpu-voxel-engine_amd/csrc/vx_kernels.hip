@@ -851,9 +851,12 @@ __device__ __forceinline__ void walk_row(SH &S, int r, uint32_t f, uint32_t t0, 
 // descriptor walk deals rows to threads round-robin, since a tile's faces
 // often come from a band of consecutive rows (a surface) that consecutive
 // ownership (4 or 16 rows per thread) would leave to a handful of threads.
+// `parts` workgroups may share a chunk (small batches): part p emits tiles
+// p, p + parts, ... and the matching share of the index stores; each part
+// repeats the row scan.
 template <int N, int MAXN, bool INTERLEAVE, class SH>
 __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
-                          SH &S) {
+                          SH &S, int part = 0, int parts = 1) {
     using R = typename RowType<MAXN>::T;
     constexpr int RPT = (N * N) / kBlock;
     const int tid = threadIdx.x;
@@ -864,7 +867,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         // drain while the rows are scanned and the face descriptors built.
         // quad::INDICES = [0,2,1,3,1,2] + 4*local face; two indices per store
         uint2 *dst = reinterpret_cast<uint2 *>(out.indices + base * 6);
-        for (uint32_t q = tid; q < 3 * F; q += kBlock) {
+        for (uint32_t q = (uint32_t)(part * kBlock + tid); q < 3 * F; q += (uint32_t)(parts * kBlock)) {
             const uint32_t face = q / 3, part = q - face * 3;
             const uint32_t b4 = 4 * face;
             uint2 v;
@@ -902,7 +905,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
     }
 
     constexpr uint32_t TILE = sizeof(S.desc) / sizeof(S.desc[0]);
-    for (uint32_t t0 = 0; t0 < F; t0 += TILE) {
+    for (uint32_t t0 = (uint32_t)part * TILE; t0 < F; t0 += (uint32_t)parts * TILE) {
         const uint32_t t1 = min(F, t0 + TILE);
         if constexpr (INTERLEAVE) {
 #pragma unroll 1
@@ -1010,16 +1013,17 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
 
 template <int N, int MAXN>
 __device__ void emit_chunk(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
-                           MeshShared<MAXN> &S) {
+                           MeshShared<MAXN> &S, int part, int parts) {
     load_rows<N, MAXN>(w, S);
     __syncthreads();
-    emit_rows<N, MAXN, (MAXN >= 32)>(w, base, F, out, S);
+    emit_rows<N, MAXN, (MAXN >= 32)>(w, base, F, out, S, part, parts);
 }
 
 template <int MAXN>
 __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint32_t F,
                                           const MeshOut &out, const MeshTables *tables,
-                                          uint32_t *overflow, MeshShared<MAXN> &S) {
+                                          uint32_t *overflow, MeshShared<MAXN> &S, int part,
+                                          int parts) {
     if (base + F > out.capacity) {
         if (threadIdx.x == 0) atomicOr(overflow, 1u);
         return;
@@ -1028,9 +1032,9 @@ __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint
     for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = threadIdx.x; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     switch (w.edge) {
-    case 16: emit_chunk<16, MAXN>(w, base, F, out, S); break;
-    case 32: if (MAXN >= 32) emit_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, base, F, out, S); break;
-    case 64: if (MAXN >= 64) emit_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(w, base, F, out, S); break;
+    case 16: emit_chunk<16, MAXN>(w, base, F, out, S, part, parts); break;
+    case 32: if (MAXN >= 32) emit_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, base, F, out, S, part, parts); break;
+    case 64: if (MAXN >= 64) emit_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(w, base, F, out, S, part, parts); break;
     default: break;
     }
 }
@@ -1045,10 +1049,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
                                                       const uint32_t *__restrict__ counts,
                                                       MeshOut out,
                                                       const MeshTables *__restrict__ tables,
-                                                      uint32_t *__restrict__ overflow) {
+                                                      uint32_t *__restrict__ overflow,
+                                                      int parts) {
     __shared__ MeshShared<MAXN> S;
-    const int item = order ? order[blockIdx.x] : (int)blockIdx.x;
-    emit_item<MAXN>(work[item], bases[item], counts[item], out, tables, overflow, S);
+    const int k = (int)blockIdx.x / parts, part = (int)blockIdx.x - k * parts;
+    const int item = order ? order[k] : k;
+    emit_item<MAXN>(work[item], bases[item], counts[item], out, tables, overflow, S, part, parts);
 }
 
 template <int MAXN>
@@ -1057,7 +1063,7 @@ __global__ void __launch_bounds__(kBlock) emit_one_kernel(MeshWork w, uint64_t b
                                                           const MeshTables *__restrict__ tables,
                                                           uint32_t *__restrict__ overflow) {
     __shared__ MeshShared<MAXN> S;
-    emit_item<MAXN>(w, base, F, out, tables, overflow, S);
+    emit_item<MAXN>(w, base, F, out, tables, overflow, S, 0, 1);
 }
 
 // --------------------------------------------------------------- fused
@@ -1420,12 +1426,16 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
                        const uint64_t *bases, const uint32_t *counts, MeshOut out,
                        const MeshTables *tables, uint32_t *overflow, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    // small batches: up to 4 workgroups per chunk, so the grid still covers
+    // about kEmitGrid workgroups and a chunk's tiles are emitted side by side
+    const int parts = n >= kEmitGrid ? 1 : std::min(4, (kEmitGrid + n - 1) / n);
+    const int grid = n * parts;
     if (max_edge <= 16)
-        emit_kernel<16><<<n, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow);
+        emit_kernel<16><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
     else if (max_edge <= 32)
-        emit_kernel<32><<<n, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow);
+        emit_kernel<32><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
     else
-        emit_kernel<64><<<n, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow);
+        emit_kernel<64><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
     return hipGetLastError();
 }
 
