@@ -268,6 +268,26 @@ def test_batch_octave_counts(ctx, oracle_mod, mode, terrain):
                     mode=mode)
 
 
+@pytest.mark.parametrize("n", [200, 600])
+def test_batch_rows_split_over_workgroups(ctx, oracle_mod, n):
+    """Staged batches of fewer than 1024 chunks fill each chunk with 4 (n <
+    512) or 2 workgroups: density terrains of 1 to 4 octaves split the
+    chunk's rows among them, other terrains are filled whole by the first.
+    Mixed edges 16/32/64 and terrains of 1, 2, 3, 4, 5 octaves and a
+    heightmap; no neighbours."""
+    say("batch split fill", n)
+    rng = np.random.default_rng(n)
+    one = V.Terrain(kind=V.TERRAIN_DENSITY, seed=13, octaves=1, frequency=1 / 24, y_scale=1 / 64)
+    pool = [one, OCT2, REF_NOISE, C2, OCT5, C1]
+    edges = [(16, 16, 16, 16, 16, 16, 32, 32, 64)[i % 9] for i in range(n)]
+    terrains = [pool[int(k)] for k in rng.integers(0, len(pool), n)]
+    cells = rng.choice(80 * 4 * 80, n, replace=False)
+    coords = [(int(c // 320) - 40, int(c // 80 % 4) - 2, int(c % 80) - 40) for c in cells]
+    b = batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1, np.int32),
+                        formats=V.FMT_VERTICES, mode=V.BATCH_STAGED)
+    b.close()
+
+
 def test_coordinates_outside_the_i32_world_are_refused(ctx, oracle_mod):
     """A chunk whose voxels or one-voxel halo would leave the i32 world range
     is refused (VX_ERR_INVALID) by every entry point that builds chunks; the

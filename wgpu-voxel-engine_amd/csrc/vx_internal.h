@@ -12,6 +12,7 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kMaxOctaves = 8;     // fBm octaves supported by the noise tables
 constexpr int kDescTile = 4096;    // face descriptors staged in LDS per emit tile
 constexpr int kEmitGrid = 1280;    // emit workgroups that fill the GPU (5 per CU)
+constexpr int kFillGrid = 1024;    // fill workgroups that fill the GPU (4 per CU)
 
 // Terrain kinds (mirror of vx_terrain.kind in include/voxel.h).
 enum : int32_t { kAir = 0, kSolid = 1, kHeightmap = 2, kDensity = 3 };
@@ -66,7 +67,7 @@ struct FillWork {
     uint8_t *ids;          // edge^3
     uint32_t *occ;         // edge^2 rows * W words (W = 1 for edge <= 32, 2 for 64)
     uint32_t *planes;      // 4 planes * edge * W words: x=0, x=N-1, z=0, z=N-1
-    uint32_t *meta;        // [0] solid block id (or kBlockMixed), [1] solid voxel count
+    uint32_t *meta;        // [0] solid block id (or kBlockMixed)
     float *noise_out;      // optional raw noise (density: edge^3, heightmap: edge^2)
 };
 

@@ -263,12 +263,15 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
 // the rare rehash.  Octaves are summed in the same order as the oracle.
 template <int N, int MAXN, bool DBG, int SEG, int OCT, class SH>
 __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
-                                  SH &S, float *noise_out) {
+                                  SH &S, float *noise_out, int seg_only = -1) {
     constexpr int L = 64 / N;
     constexpr int NSEG = N / SEG;
     static_assert(N % SEG == 0, "segments tile the column");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int G = (N + H.n + L - 1) / L;
+    // seg_only >= 0: only rows [seg_only * SEG, +SEG) (a chunk split over
+    // several workgroups)
+    const int nseg = seg_only >= 0 ? 1 : NSEG;
     float fo[OCT], ao[OCT];
     {
         float f = T.frequency, a = 1.0f;
@@ -280,9 +283,9 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
             a = a * T.gain;
         }
     }
-    for (int item = wave; item < G * NSEG; item += kBlock / 64) {
-        const int g = item / NSEG;
-        const int y0 = (item - g * NSEG) * SEG;
+    for (int item = wave; item < G * nseg; item += kBlock / 64) {
+        const int g = item / nseg;
+        const int y0 = (seg_only >= 0 ? seg_only : item - g * nseg) * SEG;
         const int li = g * L + lane / N;
         int wxi, wzi;
         // Column terms are computed per item, not hoisted: at edge 32 a lane's
@@ -466,26 +469,26 @@ __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const
     }
 }
 
-// Writes a chunk whose occupancy rows are complete in S.occ: occupancy and
-// boundary planes always, u8 ids (all solid voxels = kBlockGrass) when
-// write_ids, and meta = {solid block id, solid voxel count}.
+// Writes the layers y_lo <= y < y_hi of a chunk whose occupancy rows for
+// them are in S.occ: occupancy and boundary planes always, u8 ids (all solid
+// voxels = kBlockGrass) when write_ids; meta[0] = solid block id from the
+// workgroup that writes layer 0.
 template <int N, int MAXN, class SH>
-__device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, uint32_t solid_type) {
+__device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, uint32_t solid_type,
+                                    int y_lo = 0, int y_hi = N) {
     constexpr int W = N == 64 ? 2 : 1;
     constexpr int RS = RowStride<MAXN>::v;
-    constexpr int ROWS = N * N;
     const uint32_t *occ = reinterpret_cast<const uint32_t *>(S.occ);
     const int tid = threadIdx.x;
-    uint32_t solid = 0;
-    for (int i = tid; i < ROWS * W; i += kBlock) {
+    const int span = y_hi - y_lo;
+    for (int i = y_lo * N * W + tid; i < y_hi * N * W; i += kBlock) {
         const int r = i / W, j = i - r * W;
-        const uint32_t v = occ[r * RS + j];
-        w.occ[i] = v;
-        solid += __popc(v);
+        w.occ[i] = occ[r * RS + j];
     }
     // boundary planes: [0] x=0, [1] x=N-1 (bit z per y); [2] z=0, [3] z=N-1 (row bits)
-    for (int i = tid; i < 4 * N * W; i += kBlock) {
-        const int pl = i / (N * W), rem = i - pl * N * W, y = rem / W, j = rem - y * W;
+    for (int k = tid; k < 4 * span * W; k += kBlock) {
+        const int pl = k / (span * W), rem = k - pl * span * W, y = y_lo + rem / W;
+        const int j = rem - (y - y_lo) * W, i = pl * N * W + y * W + j;
         uint32_t v = 0;
         if (pl < 2) {
             const int xb = pl == 0 ? 0 : N - 1;
@@ -505,7 +508,7 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
         // u8 block ids, 16 voxels per 16-byte store
         constexpr int PER_ROW = N / 16;
         uint4 *ids4 = reinterpret_cast<uint4 *>(w.ids);
-        for (int q = tid; q < ROWS * PER_ROW; q += kBlock) {
+        for (int q = y_lo * N * PER_ROW + tid; q < y_hi * N * PER_ROW; q += kBlock) {
             const int r = q / PER_ROW, part = q - r * PER_ROW;
             const uint32_t word = occ[r * RS + (part >> 1)];
             const uint32_t bits = (word >> ((part & 1) * 16)) & 0xFFFFu;
@@ -518,20 +521,57 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
             ids4[q] = make_uint4(o[0], o[1], o[2], o[3]);
         }
     }
-    const uint32_t total = block_sum(solid, S.red);
-    if (tid == 0) {
-        w.meta[0] = solid_type;
-        w.meta[1] = total;
-    }
+    if (tid == 0 && y_lo == 0) w.meta[0] = solid_type;
+}
+
+// Rows [part * N / parts, +N / parts) of chunk `w` through the octave-inner
+// density loop, then their outputs (one of `parts` workgroups on the chunk).
+template <int N, int MAXN, int PARTS, int OCT>
+__device__ void fill_part(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S, int part) {
+    const Halo H{0, 0u};
+    build_ytab<N>(T, w.cy * N, S);
+    __syncthreads();
+    density_lines_oct<N, MAXN, false, N / PARTS, OCT>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
+                                                      nullptr, part);
+    __syncthreads();
+    write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass, part * (N / PARTS),
+                                 (part + 1) * (N / PARTS));
 }
 
 template <int N, int MAXN, bool DBG>
-__device__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
+__device__ __forceinline__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
     const Halo H{0, 0u};
     generate_rows<N, MAXN, DBG, (N < 32 ? N : 32), true>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
                                                          w.noise_out);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
+}
+
+// One of `parts` (2 or 4) workgroups on a chunk: density terrains with up to
+// 4 octaves split their rows among the parts; any other terrain is filled
+// whole by part 0 while the other parts return.
+template <int N, int MAXN>
+__device__ __forceinline__ void fill_chunk_split(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S,
+                                 int part, int parts) {
+    if (T.kind == kDensity && T.octaves <= 4) {
+        if (parts == 2) {
+            switch (T.octaves) {
+            case 1: fill_part<N, MAXN, 2, 1>(w, T, S, part); break;
+            case 2: fill_part<N, MAXN, 2, 2>(w, T, S, part); break;
+            case 3: fill_part<N, MAXN, 2, 3>(w, T, S, part); break;
+            default: fill_part<N, MAXN, 2, 4>(w, T, S, part); break;
+            }
+        } else {
+            switch (T.octaves) {
+            case 1: fill_part<N, MAXN, 4, 1>(w, T, S, part); break;
+            case 2: fill_part<N, MAXN, 4, 2>(w, T, S, part); break;
+            case 3: fill_part<N, MAXN, 4, 3>(w, T, S, part); break;
+            default: fill_part<N, MAXN, 4, 4>(w, T, S, part); break;
+            }
+        }
+        return;
+    }
+    if (part == 0) fill_chunk<N, MAXN, false>(w, T, S);
 }
 
 // Packs caller-provided u8 ids (already in w.ids) into occupancy rows and
@@ -567,18 +607,30 @@ __device__ void pack_chunk(const FillWork &w, FillShared<MAXN> &S) {
     write_chunk_outputs<N, MAXN>(w, S, false, st);
 }
 
-template <int MAXN>
+// SPLIT: the workgroup is part `part` of `parts` on this chunk.
+template <int MAXN, bool SPLIT = false>
 __device__ __forceinline__ void fill_item(const FillWork &w, const TerrainDev *terrains,
-                                          const GradLut *lut, FillShared<MAXN> &S) {
+                                          const GradLut *lut, FillShared<MAXN> &S, int part = 0,
+                                          int parts = 1) {
     const TerrainDev &T = terrains[w.terrain];
     for (int i = threadIdx.x; i < 512; i += kBlock) S.perm[i] = T.perm[i];
     for (int i = threadIdx.x; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
     __syncthreads();
     if (w.noise_out) {  // parity-check path: also stores the raw noise
+        if (SPLIT && part > 0) return;
         switch (w.edge) {
         case 16: fill_chunk<16, MAXN, true>(w, T, S); break;
         case 32: if (MAXN >= 32) fill_chunk<(MAXN >= 32 ? 32 : 16), MAXN, true>(w, T, S); break;
         case 64: if (MAXN >= 64) fill_chunk<(MAXN >= 64 ? 64 : 16), MAXN, true>(w, T, S); break;
+        default: break;
+        }
+        return;
+    }
+    if (SPLIT) {
+        switch (w.edge) {
+        case 16: fill_chunk_split<16, MAXN>(w, T, S, part, parts); break;
+        case 32: if (MAXN >= 32) fill_chunk_split<(MAXN >= 32 ? 32 : 16), MAXN>(w, T, S, part, parts); break;
+        case 64: if (MAXN >= 64) fill_chunk_split<(MAXN >= 64 ? 64 : 16), MAXN>(w, T, S, part, parts); break;
         default: break;
         }
         return;
@@ -601,6 +653,19 @@ fill_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order
     __shared__ FillShared<MAXN> S;
     const int item = order ? order[blockIdx.x] : (int)blockIdx.x;
     fill_item<MAXN>(work[item], terrains, lut, S);
+}
+
+// Small batches: `parts` workgroups per chunk (a kernel of its own so that
+// the row-split path does not add to the registers of fill_kernel).
+template <int MAXN>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+fill_split_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order,
+                  const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut,
+                  int parts) {
+    __shared__ FillShared<MAXN> S;
+    const int k = (int)blockIdx.x / parts, part = (int)blockIdx.x - k * parts;
+    const int item = order ? order[k] : k;
+    fill_item<MAXN, true>(work[item], terrains, lut, S, part, parts);
 }
 
 template <int MAXN>
@@ -1370,9 +1435,20 @@ __global__ void __launch_bounds__(kBlock) draw_write_kernel(Planes planes,
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const TerrainDev *terrains, const GradLut *lut, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (max_edge <= 16) fill_kernel<16><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
-    else if (max_edge <= 32) fill_kernel<32><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
-    else fill_kernel<64><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
+    // small batches: 2 or 4 workgroups per chunk, so the grid still covers
+    // about kFillGrid workgroups (density terrains with up to 4 octaves split
+    // their rows; other chunks are filled by the first of their workgroups)
+    const int parts = n >= kFillGrid ? 1 : (2 * n >= kFillGrid ? 2 : 4);
+    if (parts == 1) {
+        if (max_edge <= 16) fill_kernel<16><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
+        else if (max_edge <= 32) fill_kernel<32><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
+        else fill_kernel<64><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
+        return hipGetLastError();
+    }
+    const int grid = n * parts;
+    if (max_edge <= 16) fill_split_kernel<16><<<grid, kBlock, 0, s>>>(work, order, terrains, lut, parts);
+    else if (max_edge <= 32) fill_split_kernel<32><<<grid, kBlock, 0, s>>>(work, order, terrains, lut, parts);
+    else fill_split_kernel<64><<<grid, kBlock, 0, s>>>(work, order, terrains, lut, parts);
     return hipGetLastError();
 }
 
