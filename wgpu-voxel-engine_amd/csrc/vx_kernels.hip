@@ -1502,9 +1502,12 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
                        const uint64_t *bases, const uint32_t *counts, MeshOut out,
                        const MeshTables *tables, uint32_t *overflow, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    // small batches: up to 4 workgroups per chunk, so the grid still covers
-    // about kEmitGrid workgroups and a chunk's tiles are emitted side by side
-    const int parts = n >= kEmitGrid ? 1 : std::min(4, (kEmitGrid + n - 1) / n);
+    // small batches: up to 4 workgroups per chunk (8 when the batch holds
+    // 64^3 chunks, which reach 12 descriptor tiles of faces), so the grid
+    // still covers about kEmitGrid workgroups and a chunk's tiles are
+    // emitted side by side
+    const int parts =
+        n >= kEmitGrid ? 1 : std::min(max_edge > 32 ? 8 : 4, (kEmitGrid + n - 1) / n);
     const int grid = n * parts;
     if (max_edge <= 16)
         emit_kernel<16><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
