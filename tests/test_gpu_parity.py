@@ -268,12 +268,13 @@ def test_batch_octave_counts(ctx, oracle_mod, mode, terrain):
                     mode=mode)
 
 
-@pytest.mark.parametrize("n", [200, 600])
+@pytest.mark.parametrize("n", [40, 200, 600])
 def test_batch_rows_split_over_workgroups(ctx, oracle_mod, n):
-    """Staged batches of fewer than 1024 chunks fill each chunk with 4 (n <
-    512) or 2 workgroups: density terrains of 1 to 4 octaves split the
-    chunk's rows among them, other terrains are filled whole by the first.
-    Mixed edges 16/32/64 and terrains of 1, 2, 3, 4, 5 octaves and a
+    """Staged batches of fewer than 1024 chunks fill each chunk with 8 (at
+    most 128 chunks, 64^3 chunks present: edge 64 in 8 parts, smaller edges
+    in 4), 4 (n < 512) or 2 workgroups: density terrains of 1 to 4 octaves
+    split the chunk's rows among them, other terrains are filled whole by the
+    first.  Mixed edges 16/32/64 and terrains of 1, 2, 3, 4, 5 octaves and a
     heightmap; no neighbours."""
     say("batch split fill", n)
     rng = np.random.default_rng(n)

@@ -547,14 +547,26 @@ __device__ __forceinline__ void fill_chunk(const FillWork &w, const TerrainDev &
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
 }
 
-// One of `parts` (2 or 4) workgroups on a chunk: density terrains with up to
-// 4 octaves split their rows among the parts; any other terrain is filled
-// whole by part 0 while the other parts return.
+// One of `parts` (2, 4 or 8) workgroups on a chunk: density terrains with
+// up to 4 octaves split their rows among the parts (8 only at edge 64; a
+// smaller chunk uses the first 4); any other terrain is filled whole by part
+// 0 while the other parts return.
 template <int N, int MAXN>
 __device__ __forceinline__ void fill_chunk_split(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S,
                                  int part, int parts) {
     if (T.kind == kDensity && T.octaves <= 4) {
-        if (parts == 2) {
+        if (parts == 8 && N < 64) {
+            if (part >= 4) return;
+            parts = 4;
+        }
+        if (parts == 8) {
+            switch (T.octaves) {
+            case 1: fill_part<N, MAXN, (N == 64 ? 8 : 4), 1>(w, T, S, part); break;
+            case 2: fill_part<N, MAXN, (N == 64 ? 8 : 4), 2>(w, T, S, part); break;
+            case 3: fill_part<N, MAXN, (N == 64 ? 8 : 4), 3>(w, T, S, part); break;
+            default: fill_part<N, MAXN, (N == 64 ? 8 : 4), 4>(w, T, S, part); break;
+            }
+        } else if (parts == 2) {
             switch (T.octaves) {
             case 1: fill_part<N, MAXN, 2, 1>(w, T, S, part); break;
             case 2: fill_part<N, MAXN, 2, 2>(w, T, S, part); break;
@@ -1435,10 +1447,14 @@ __global__ void __launch_bounds__(kBlock) draw_write_kernel(Planes planes,
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const TerrainDev *terrains, const GradLut *lut, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    // small batches: 2 or 4 workgroups per chunk, so the grid still covers
-    // about kFillGrid workgroups (density terrains with up to 4 octaves split
-    // their rows; other chunks are filled by the first of their workgroups)
-    const int parts = n >= kFillGrid ? 1 : (2 * n >= kFillGrid ? 2 : 4);
+    // small batches: 2, 4 or 8 workgroups per chunk, so the grid still
+    // covers about kFillGrid workgroups (density terrains with up to 4
+    // octaves split their rows; other chunks are filled by the first of
+    // their workgroups); 8 only for batches of at most 128 chunks that hold
+    // 64^3 chunks, whose parts then fill 8 rows each
+    const int parts = n >= kFillGrid ? 1
+                      : 2 * n >= kFillGrid ? 2
+                      : (max_edge > 32 && 8 * n <= kFillGrid) ? 8 : 4;
     if (parts == 1) {
         if (max_edge <= 16) fill_kernel<16><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
         else if (max_edge <= 32) fill_kernel<32><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
