@@ -846,8 +846,17 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int per = (n + 1023) / 1024;
     const int b = tid * per, e = min(n, b + per);
+    // a thread's first kCached counts stay in registers for the three passes
+    // (all of them up to 8,192 chunks); the rest are read again
+    constexpr int kCached = 8;
+    uint32_t cv[kCached];
+#pragma unroll
+    for (int j = 0; j < kCached; j++) cv[j] = b + j < e ? counts[b + j] : 0u;
+    const int ec = min(e, b + kCached);
     uint64_t local = 0;
-    for (int i = b; i < e; i++) local += counts[i];
+#pragma unroll
+    for (int j = 0; j < kCached; j++) local += cv[j];
+    for (int i = b + kCached; i < e; i++) local += counts[i];
     uint64_t incl = local;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -862,7 +871,12 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
         sum += s_wave[i];
     }
     uint64_t run = wbase + incl - local;
-    for (int i = b; i < e; i++) {
+#pragma unroll
+    for (int j = 0; j < kCached; j++) {
+        if (b + j < ec) bases[b + j] = run;
+        run += cv[j];
+    }
+    for (int i = b + kCached; i < e; i++) {
         bases[i] = run;
         run += counts[i];
     }
@@ -874,7 +888,10 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     static_assert(kOrderBuckets == 1024, "one bucket per thread");
     s_hist[tid] = 0u;
     __syncthreads();
-    for (int i = b; i < e; i++) atomicAdd(&s_hist[order_bucket(counts[i])], 1u);
+#pragma unroll
+    for (int j = 0; j < kCached; j++)
+        if (b + j < ec) atomicAdd(&s_hist[order_bucket(cv[j])], 1u);
+    for (int i = b + kCached; i < e; i++) atomicAdd(&s_hist[order_bucket(counts[i])], 1u);
     __syncthreads();
     const uint32_t cnt = s_hist[tid];
     uint32_t cincl = cnt;
@@ -890,7 +907,11 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     for (int i = 0; i < wave; i++) cbase += s_w32[i];
     s_hist[tid] = cbase + cincl - cnt;
     __syncthreads();
-    for (int i = b; i < e; i++) order_out[atomicAdd(&s_hist[order_bucket(counts[i])], 1u)] = i;
+#pragma unroll
+    for (int j = 0; j < kCached; j++)
+        if (b + j < ec) order_out[atomicAdd(&s_hist[order_bucket(cv[j])], 1u)] = b + j;
+    for (int i = b + kCached; i < e; i++)
+        order_out[atomicAdd(&s_hist[order_bucket(counts[i])], 1u)] = i;
 }
 
 // Descriptors (row | x << 12 | dir << 18) of row r's faces that fall in the
