@@ -391,7 +391,7 @@ def main():
     chosen_ms = b.profile(args.profile_reps)
 
     N3, N2 = EDGE ** 3, EDGE ** 2
-    fill_bytes = n * (N3 + N3 // 8 + 4 * N2 // 8 + 8)          # ids + occupancy + planes + meta
+    fill_bytes = n * (N3 + N3 // 8 + 4 * N2 // 8 + 4)          # ids + occupancy + planes + meta
     fill_flops = n * N3 * t.octaves * K_EVAL
     emit_bytes = n * (N3 // 8 + 4 * N2 // 8 + 4 + 8) + 104 * total_faces
     fused_bytes = fill_bytes + 104 * total_faces + n * 12       # one pass: writes only

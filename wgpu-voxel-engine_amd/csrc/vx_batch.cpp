@@ -7,7 +7,7 @@
 //
 // Layout in HBM
 //   store   per chunk i at chunk_off[i]: ids (N^3 B) | occupancy rows
-//           (N^2 * W u32) | 4 boundary planes (4 * N * W u32) | meta (2 u32),
+//           (N^2 * W u32) | 4 boundary planes (4 * N * W u32) | meta (solid block id, u32),
 //           each 256-byte aligned.
 //   ctl     counts[n] u32 | bases[n] u64 | total u64 | overflow u32
 //   arena   instances (68 B/face) | vertices (80 B/face) | indices (24 B/face)
