@@ -296,6 +296,38 @@ def measured_traffic(kernel: str, faces: int):
     return measured_counter(kernel, faces, "bytes")
 
 
+def two_stream_leg(t, coords, nb, b, ctx, faces: int, steps: int):
+    """The same 4096-chunk batch twice, each on its own context (its own HIP
+    stream), replayed alternately: one batch's emit (HBM stores) overlaps the
+    other's fill (FP32 issue).  Reported beside `value`, which times one
+    batch at a time on one stream."""
+    ctx2 = V.Context(0)
+    try:
+        b2 = V.Batch(ctx2, coords, EDGE, ctx2.terrain(t), nb, formats=V.FMT_VERTICES)
+        b2.run()
+        assert b2.wait() == faces
+        b2.autotune(3)
+        for _ in range(2):
+            b.run()
+            b2.run()
+        ctx.synchronize()
+        ctx2.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run()
+            b2.run()
+        ctx.synchronize()
+        ctx2.synchronize()
+        el = time.perf_counter() - t0
+        assert b.wait() == faces and b2.wait() == faces
+        b2.close()
+    finally:
+        ctx2.close()
+    batches = 2 * steps
+    return {"streams": 2, "batches": batches, "ms_per_batch": round(el * 1e3 / batches, 4),
+            "chunks_per_s": round(batches * len(coords) / el, 1)}
+
+
 def serving_leg(rates=(2000, 5000, 10000, 20000), duration=1.5):
     """Streaming (config 4): camera random walk at 20 blocks/s (seed 3),
     render radius 16 chunks of 32^3, Poisson arrivals, sizes 16/32/64
@@ -334,6 +366,8 @@ def main():
     ap.add_argument("--no-frame", action="store_true", help="skip the full-frame leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the streaming leg")
     ap.add_argument("--no-world", action="store_true", help="skip the tick-by-tick world leg")
+    ap.add_argument("--no-two", action="store_true",
+                    help="skip the two-batches-in-flight leg (two streams)")
     ap.add_argument("--pmc-run", choices=["fused", "staged"], default=None,
                     help="counter-collection run: 2 runs of the batch in one form, then a "
                          "256 MiB plain copy kernel (the byte-counter reference); no timing")
@@ -471,6 +505,12 @@ def main():
                  "launches_per_step": 3 * n + 1}
         log(f"[bench] plain per-chunk path: {plain['value']:.0f} chunks/s")
 
+    two = None
+    if not args.no_two:
+        two = two_stream_leg(t, coords, nb, b, ctx, total_faces, args.steps)
+        log(f"[bench] two batches in flight: {two['ms_per_batch']} ms per batch, "
+            f"{two['chunks_per_s']:.0f} chunks/s")
+
     frame = None
     if not args.no_frame:
         log("[bench] full frame: 65,536 resident + 512 new chunks per frame + cull...")
@@ -522,6 +562,7 @@ def main():
         "autotune": tune,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "two_batches_in_flight": two,
         "other_form_gpu": other,
         "full_frame": frame,
         "other_configs": configs,
