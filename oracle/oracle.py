@@ -85,6 +85,7 @@ def lib():
             "vxo_frustum_planes": (None, [P, P]),
             "vxo_frustum_contains": (C.c_int, [P, i32, i32, i32, i32]),
             "vxo_batch_run": (i64, [C.POINTER(Terrain), P, P, i32, i32, C.c_int, P]),
+            "vxo_batch_mesh": (i64, [C.POINTER(Terrain), P, P, i32, i32, C.c_int, P, P, P, i64, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -241,6 +242,26 @@ def batch_run(t: Terrain, coords: np.ndarray, nb: np.ndarray | None, edge: int, 
     faces = np.zeros(n, np.int64)
     total = lib().vxo_batch_run(C.byref(t), _p(coords), nbp, n, edge, threads, _p(faces))
     return total, faces
+
+
+def batch_mesh(t: Terrain, coords: np.ndarray, nb: np.ndarray | None, edge: int, threads: int,
+               bases: np.ndarray, total_faces: int):
+    """The batch driver keeping its meshes: vertex and index buffers of
+    `total_faces` faces with chunk i's faces from bases[i], and the face
+    count per chunk.  Raises if a chunk's range falls outside the buffers."""
+    coords = np.ascontiguousarray(coords, np.int32).reshape(-1, 3)
+    n = len(coords)
+    bases = np.ascontiguousarray(bases, np.int64)
+    assert len(bases) == n
+    nbp = None if nb is None else _p(np.ascontiguousarray(nb, np.int32))
+    vert = np.zeros(4 * total_faces, VERTEX_DTYPE)
+    idx = np.zeros(6 * total_faces, np.uint32)
+    faces = np.zeros(n, np.int64)
+    total = lib().vxo_batch_mesh(C.byref(t), _p(coords), nbp, n, edge, threads, _p(bases),
+                                 _p(vert), _p(idx), total_faces, _p(faces))
+    if total < 0:
+        raise ValueError("a chunk's faces fall outside the buffers for these bases")
+    return total, faces, vert, idx
 
 
 def expand(faces: np.ndarray):

@@ -622,6 +622,14 @@ typedef struct {
     int phase;
     atomic_llong total;
     int64_t *faces_out;
+    /* optional mesh output: chunk i's vertices and indices at face bases[i]
+       of vert_out / idx_out (cap_faces faces); a chunk that would not fit
+       sets overflow instead */
+    const int64_t *bases;
+    vxo_vertex *vert_out;
+    uint32_t *idx_out;
+    int64_t cap_faces;
+    atomic_int overflow;
 } batch_job;
 
 static void *batch_worker(void *arg) {
@@ -651,8 +659,17 @@ static void *batch_worker(void *arg) {
                 adj[k] = j >= 0 ? J->ids + (size_t)j * vox : NULL;
             }
             int64_t nf = vxo_mesh_faces(ids, N, cx * N, cy * N, cz * N, adj, faces, fmax);
-            vxo_vertex *vb = (vxo_vertex *)malloc((size_t)(nf > 0 ? nf : 1) * 4 * sizeof(vxo_vertex));
-            uint32_t *ib = (uint32_t *)malloc((size_t)(nf > 0 ? nf : 1) * 6 * sizeof(uint32_t));
+            const int keep = J->vert_out != NULL;
+            if (keep && (J->bases[i] < 0 || J->bases[i] + nf > J->cap_faces)) {
+                atomic_store(&J->overflow, 1);
+                if (J->faces_out) J->faces_out[i] = nf;
+                atomic_fetch_add(&J->total, (long long)nf);
+                continue;
+            }
+            vxo_vertex *vb = keep ? J->vert_out + 4 * J->bases[i]
+                                  : (vxo_vertex *)malloc((size_t)(nf > 0 ? nf : 1) * 4 * sizeof(vxo_vertex));
+            uint32_t *ib = keep ? J->idx_out + 6 * J->bases[i]
+                                : (uint32_t *)malloc((size_t)(nf > 0 ? nf : 1) * 6 * sizeof(uint32_t));
             for (int64_t f = 0; f < nf; f++) {
                 vxo_instance inst;
                 vxo_instance_of(&faces[f], &inst);
@@ -662,8 +679,10 @@ static void *batch_worker(void *arg) {
             /* keep the writes observable so they are not optimised away */
             volatile uint32_t sink = nf > 0 ? ib[6 * nf - 1] : 0;
             (void)sink;
-            free(vb);
-            free(ib);
+            if (!keep) {
+                free(vb);
+                free(ib);
+            }
             if (J->faces_out) J->faces_out[i] = nf;
             atomic_fetch_add(&J->total, (long long)nf);
         }
@@ -673,8 +692,10 @@ static void *batch_worker(void *arg) {
     return NULL;
 }
 
-int64_t vxo_batch_run(const vxo_terrain *t, const int32_t *coords, const int32_t *nb, int32_t n,
-                      int32_t edge, int threads, int64_t *faces_out) {
+static int64_t batch_drive(const vxo_terrain *t, const int32_t *coords, const int32_t *nb,
+                           int32_t n, int32_t edge, int threads, int64_t *faces_out,
+                           const int64_t *bases, vxo_vertex *vert_out, uint32_t *idx_out,
+                           int64_t cap_faces) {
     if (n <= 0) return 0;
     if (threads < 1) threads = 1;
     batch_job J;
@@ -685,6 +706,11 @@ int64_t vxo_batch_run(const vxo_terrain *t, const int32_t *coords, const int32_t
     J.edge = edge;
     J.ids = (uint8_t *)malloc((size_t)n * edge * edge * edge);
     J.faces_out = faces_out;
+    J.bases = bases;
+    J.vert_out = vert_out;
+    J.idx_out = idx_out;
+    J.cap_faces = cap_faces;
+    atomic_init(&J.overflow, 0);
     atomic_init(&J.total, 0);
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
     for (int phase = 0; phase < 2; phase++) {
@@ -695,5 +721,19 @@ int64_t vxo_batch_run(const vxo_terrain *t, const int32_t *coords, const int32_t
     }
     free(th);
     free(J.ids);
+    if (atomic_load(&J.overflow)) return -1;
     return (int64_t)atomic_load(&J.total);
+}
+
+int64_t vxo_batch_run(const vxo_terrain *t, const int32_t *coords, const int32_t *nb, int32_t n,
+                      int32_t edge, int threads, int64_t *faces_out) {
+    return batch_drive(t, coords, nb, n, edge, threads, faces_out, NULL, NULL, NULL, 0);
+}
+
+int64_t vxo_batch_mesh(const vxo_terrain *t, const int32_t *coords, const int32_t *nb, int32_t n,
+                       int32_t edge, int threads, const int64_t *bases, vxo_vertex *vert_out,
+                       uint32_t *idx_out, int64_t cap_faces, int64_t *faces_out) {
+    if (!bases || !vert_out || !idx_out) return -1;
+    return batch_drive(t, coords, nb, n, edge, threads, faces_out, bases, vert_out, idx_out,
+                       cap_faces);
 }

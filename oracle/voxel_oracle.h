@@ -170,6 +170,13 @@ int vxo_frustum_contains(const float planes[24], int32_t cx, int32_t cy, int32_t
  * keeping). */
 int64_t vxo_batch_run(const vxo_terrain *t, const int32_t *coords, const int32_t *nb, int32_t n,
                       int32_t edge, int threads, int64_t *faces_out);
+/* The same batch, keeping the meshes: chunk i's 4 vertices and 6 chunk-local
+   indices per face written from face bases[i] of vert_out / idx_out, which
+   hold cap_faces faces.  Returns the total faces, or -1 when a chunk's range
+   falls outside [0, cap_faces) (nothing is written for that chunk). */
+int64_t vxo_batch_mesh(const vxo_terrain *t, const int32_t *coords, const int32_t *nb, int32_t n,
+                       int32_t edge, int threads, const int64_t *bases, vxo_vertex *vert_out,
+                       uint32_t *idx_out, int64_t cap_faces, int64_t *faces_out);
 
 #ifdef __cplusplus
 }

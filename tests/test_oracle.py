@@ -401,3 +401,33 @@ def test_batch_driver_counts(oracle_mod):
         adj = tuple(ids[j] if j >= 0 else None for j in nb[i])
         assert per[i] == len(oracle_mod.mesh_faces(ids[i], 16, tuple(c * 16), adj))
     assert total == per.sum()
+
+
+def test_batch_driver_meshes_match_per_chunk(oracle_mod):
+    """The driver's kept meshes (the full-size GPU comparison's reference)
+    equal the per-chunk restatement at the bases given, in a shuffled order
+    of bases; bases that do not fit are refused."""
+    import voxel_amd as V  # host helper only: builds the neighbour table
+    O = oracle_mod
+    t = O.terrain()
+    coords = np.array([(x, 0, z) for x in range(3) for z in range(3)], np.int32)
+    nb = V.grid_neighbours(3, 3)
+    total, per = O.batch_run(t, coords, nb, 16, 4)
+    # lay the chunks out in reverse request order to show bases are honoured
+    order = np.arange(len(coords))[::-1]
+    bases = np.zeros(len(coords), np.int64)
+    run = 0
+    for i in order:
+        bases[i] = run
+        run += per[i]
+    tot2, per2, vert, idx = O.batch_mesh(t, coords, nb, 16, 4, bases, int(total))
+    assert tot2 == total and (per2 == per).all()
+    ids = [O.chunk_fill(t, *c, 16) for c in coords]
+    for i, c in enumerate(coords):
+        adj = tuple(ids[j] if j >= 0 else None for j in nb[i])
+        _, _, v, ix = O.mesh_buffers(ids[i], 16, tuple(c * 16), adj)
+        f0, n = int(bases[i]), int(per[i])
+        assert (vert[4 * f0:4 * (f0 + n)].view(np.uint32) == v.view(np.uint32)).all()
+        assert (idx[6 * f0:6 * (f0 + n)] == ix).all()
+    with pytest.raises(ValueError):
+        O.batch_mesh(t, coords, nb, 16, 4, bases, int(total) - 1)
