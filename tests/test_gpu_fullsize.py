@@ -65,7 +65,7 @@ def test_config2_full_grid(ctx, oracle_mod, mode):
 def test_config3_full_ragged(ctx, oracle_mod, mode):
     """2048 chunks, edges uniform over {16, 32, 64}, classes 30% air / 10%
     solid / 60% 4-octave heightmap surface (one seeded stream, seed 11), no
-    neighbours: counts of all chunks, byte-exact sample, prefix order.  The
+    neighbours: every chunk's faces byte for byte, prefix order.  The
     15.0 M faces exceed the default arena (4 N^2 per chunk, 14.5 M), so the
     run also exercises the grow-and-re-emit path after either form."""
     O = oracle_mod
@@ -105,26 +105,27 @@ def test_config3_full_ragged(ctx, oracle_mod, mode):
             assert got[i] == 0
         elif cls[i] == 1:
             assert got[i] == 6 * edges[i] ** 2
-    # every surface chunk's count against the oracle, 48 of them byte for byte
+    # every surface and solid chunk byte for byte against the oracle (air
+    # chunks have no faces, checked above): the whole 15.0 M-face mesh
     verts, idxs = b.mesh(V.FMT_VERTICES), b.mesh(V.FMT_INDICES)
-    surf = np.nonzero(cls == 2)[0]
-    sample = set(rng.choice(surf, 48, replace=False).tolist())
-    for i in surf:
-        t = oterrain(O, terrains[(2, int(edges[i]))][0])
+    vbits = bits(verts)
+    for i in np.nonzero(cls != 0)[0]:
+        t = oterrain(O, terrains[(int(cls[i]), int(edges[i]))][0])
         e = int(edges[i])
         ids = O.chunk_fill(t, *map(int, coords[i]), e)
-        if i in sample:
-            faces, _, vert, ix = O.mesh_buffers(ids, e, tuple(int(v) * e for v in coords[i]))
-            assert (bits(verts[4 * firsts[i]:4 * (firsts[i] + got[i])]) == bits(vert)).all(), i
-            assert (idxs[6 * firsts[i]:6 * (firsts[i] + got[i])] == ix).all(), i
-        else:
-            faces = O.mesh_faces(ids, e, tuple(int(v) * e for v in coords[i]))
+        faces, _, vert, ix = O.mesh_buffers(ids, e, tuple(int(v) * e for v in coords[i]))
         assert got[i] == len(faces), i
+        f0 = int(firsts[i])
+        assert np.array_equal(vbits[20 * f0:20 * (f0 + got[i])], bits(vert)), i
+        assert np.array_equal(idxs[6 * f0:6 * (f0 + got[i])], ix), i
 
 
 def test_config5_full_resident_grid_and_cull(ctx, oracle_mod):
     """65,536 resident chunks (256 x 256, seed 5): face counts of all chunks
-    against the oracle, and the cull mask of every AABB for three cameras."""
+    against the oracle; the whole mesh (183 M faces: 14.7 GB of vertices,
+    past every 32-bit byte offset, and 4.4 GB of indices) byte for byte
+    against the oracle driver's meshes at the same bases, compared in slices;
+    and the cull mask of every AABB for three cameras."""
     import test_oracle as T
     O = oracle_mod
     say("config 5 full")
@@ -138,6 +139,20 @@ def test_config5_full_resident_grid_and_cull(ctx, oracle_mod):
     otot, ofaces = O.batch_run(oterrain(O, C5), coords, nb, 32, THREADS)
     got = np.array([b.chunk_faces(i)[1] for i in range(len(coords))])
     assert total == otot and (got == ofaces).all()
+    firsts = np.array([b.chunk_faces(i)[0] for i in range(len(coords))])
+    assert (firsts == np.concatenate([[0], np.cumsum(got)[:-1]])).all()
+    _, _, overt, oidx = O.batch_mesh(oterrain(O, C5), coords, nb, 32, THREADS, firsts, total)
+    step = 1 << 28
+    gv, ov = bits(b.mesh(V.FMT_VERTICES)), bits(overt)
+    assert len(gv) == len(ov) == 20 * total
+    for k in range(0, len(gv), step):
+        assert np.array_equal(gv[k:k + step], ov[k:k + step]), k // 20
+    del gv, ov, overt
+    gi = b.mesh(V.FMT_INDICES)
+    assert len(gi) == len(oidx) == 6 * total
+    for k in range(0, len(gi), step):
+        assert np.array_equal(gi[k:k + step], oidx[k:k + step]), k // 6
+    del gi, oidx
     cb = V.DeviceBuffer.of(ctx, coords)
     mb = V.DeviceBuffer(ctx, len(coords))
     rng = np.random.default_rng(5)
