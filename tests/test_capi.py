@@ -204,3 +204,21 @@ def test_header_is_plain_c_and_links(tmp_path):
         st = getattr(V, pyname)
         assert sizes[cname] == (C.sizeof(st), C.alignment(st)), cname
     assert sizes["vx_draw_indexed"][0] == V.DRAW_DTYPE.itemsize == 20
+
+
+def test_missing_library_fails_loudly():
+    """With the HIP library absent, the binding raises on first use (no CPU
+    fallback exists for any entry point)."""
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import voxel_amd as V\n"
+            "V.LIB_PATH = '/nonexistent/libvoxel_hip.so'\n"
+            "try:\n"
+            "    V.Context(0)\n"
+            "except RuntimeError as e:\n"
+            "    print('raised:', e)\n"
+            "    sys.exit(0)\n"
+            "sys.exit(1)\n") % os.path.dirname(V.__file__)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "is missing" in r.stdout
