@@ -103,8 +103,7 @@ def frame_leg(ctx, faces_per_chunk: float, frames: int = 20):
     grid; their -x neighbours resident, +-z in the same batch) meshed and the
     65,536 resident AABBs culled against a moving seeded camera (fov 70 deg,
     near 0.1, far 1024) in one launch."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import test_oracle as T  # camera helpers only (look_at / perspective)
+    import camera as T  # the package's camera matrices (look_at / perspective)
 
     t5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64, lacunarity=2.0,
                    gain=0.5, y_scale=1 / 128)
@@ -198,8 +197,7 @@ def world_leg(ticks: int = 130):
     a scripted walk (move, stand while the loads drain and rebuilds run, turn,
     jump far, stand), render distance 5, results receivable one tick after
     submission, fBm seed 5."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import test_oracle as T  # camera helpers only
+    import camera as T  # the package's camera matrices
 
     with V.Context(0) as c:
         t5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64,
@@ -227,19 +225,10 @@ def world_leg(ticks: int = 130):
             "unloaded": st["unloaded"], "resident_chunks": st["chunks"]}
 
 
-def configs_leg(ctx, reps: int = 20):
-    """The other batch configurations, each one captured graph replayed
-    `reps` times after a sizing run: config 1 (4 x 4 chunks of 16^3, 2-D
-    heightmap seed 42, 1 octave, f 1/32, heights 0..15, in-grid neighbours)
-    and config 3 (2048 chunks, edges uniform over 16/32/64, classes 30% air /
-    10% solid / 60% 4-octave heightmap surface with height = edge, one
-    seeded stream, seed 11, no neighbours)."""
-    out = {}
-    c1 = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=42, octaves=1, frequency=1 / 32,
-                   height_scale=16.0, height_max=15)
-    coords1 = [(x, 0, z) for x in range(4) for z in range(4)]
-    b1 = V.Batch(ctx, coords1, 16, ctx.terrain(c1), V.grid_neighbours(4, 4),
-                 formats=V.FMT_VERTICES)
+def config3_batch(ctx):
+    """Config 3: 2048 chunks, edges uniform over 16/32/64, classes 30% air /
+    10% solid / 60% 4-octave heightmap surface with height = edge, one seeded
+    stream (seed 11), no neighbours."""
     rng = np.random.default_rng(11)
     n3 = 2048
     edges = rng.choice([16, 32, 64], size=n3)
@@ -256,6 +245,23 @@ def configs_leg(ctx, reps: int = 20):
     coords3 = [(i % 64, 0, i // 64) for i in range(n3)]
     b3 = V.Batch(ctx, coords3, edges, [tids[(int(k), int(e))] for e, k in zip(edges, cls)],
                  np.full((n3, 4), -1, np.int32), formats=V.FMT_VERTICES)
+    return b3, n3
+
+
+def configs_leg(ctx, reps: int = 20):
+    """The other batch configurations, each one captured graph replayed
+    `reps` times after a sizing run: config 1 (4 x 4 chunks of 16^3, 2-D
+    heightmap seed 42, 1 octave, f 1/32, heights 0..15, in-grid neighbours)
+    and config 3 (2048 chunks, edges uniform over 16/32/64, classes 30% air /
+    10% solid / 60% 4-octave heightmap surface with height = edge, one
+    seeded stream, seed 11, no neighbours)."""
+    out = {}
+    c1 = V.Terrain(kind=V.TERRAIN_HEIGHTMAP, seed=42, octaves=1, frequency=1 / 32,
+                   height_scale=16.0, height_max=15)
+    coords1 = [(x, 0, z) for x in range(4) for z in range(4)]
+    b1 = V.Batch(ctx, coords1, 16, ctx.terrain(c1), V.grid_neighbours(4, 4),
+                 formats=V.FMT_VERTICES)
+    b3, n3 = config3_batch(ctx)
     for name, b, n in (("config1", b1, len(coords1)), ("config3", b3, n3)):
         b.run()
         faces = b.wait()  # sizes the arena (config 3 outgrows the default once)
@@ -328,19 +334,57 @@ def two_stream_leg(t, coords, nb, b, ctx, faces: int, steps: int):
             "chunks_per_s": round(batches * len(coords) / el, 1)}
 
 
-def serving_leg(rates=(2000, 5000, 10000, 20000), duration=1.5):
+def instances_leg(ctx, t, coords, nb, steps: int):
+    """The same 4096-chunk batch emitting InstanceRaw records only (68 B per
+    face, src/renderer/instance.rs:1-6), the format the Rust renderer
+    consumes: graph replays timed like `value`, the emit kernel timed with
+    HIP events."""
+    b = V.Batch(ctx, coords, EDGE, ctx.terrain(t), nb, formats=V.FMT_INSTANCES)
+    b.run()
+    faces = b.wait()
+    b.set_mode(V.BATCH_STAGED)
+    b.run()
+    b.wait()
+    for _ in range(3):
+        b.run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    assert b.wait() == faces
+    ms = b.profile(5)
+    n = len(coords)
+    N3, N2 = EDGE ** 3, EDGE ** 2
+    emit_bytes = n * (N3 // 8 + 4 * N2 // 8 + 4 + 8) + 68 * faces
+    b.close()
+    return {"format": "InstanceRaw (68 B/face)", "faces": faces,
+            "ms_per_step": round(el * 1e3 / steps, 4), "chunks_per_s": round(n * steps / el, 1),
+            "emit_ms": round(float(ms[3]), 4),
+            "emit_GB/s": round(emit_bytes / (ms[3] * 1e-3) / 1e9, 1),
+            "emit_algorithmic_bytes": emit_bytes}
+
+
+def serving_leg(rates=(2000, 5000, 10000, 20000), duration=1.5,
+                pressure=((20000, 0.25), (20000, 0.125))):
     """Streaming (config 4): camera random walk at 20 blocks/s (seed 3),
     render radius 16 chunks of 32^3, Poisson arrivals, sizes 16/32/64
-    uniform, 8 GB HBM admission budget with LRU eviction."""
+    uniform, 8 GB HBM admission budget with LRU eviction.  In 1.5 s the
+    working set (about 1 GB) stays under 8 GB, so the `pressure` runs repeat
+    the top rate under budgets of 0.25 and 0.125 GB, where LRU admission
+    evicts throughout."""
     out = []
-    with V.Context(0, hbm_budget=8 << 30) as c:
-        t = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
-                      gain=0.5, y_scale=1 / 128)
-        tid = c.terrain(t)
-        for rate in rates:
+    runs = [(rate, 8.0) for rate in rates] + list(pressure)
+    for rate, budget_gb in runs:
+        with V.Context(0, hbm_budget=int(budget_gb * 2**30)) as c:
+            t = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64,
+                          lacunarity=2.0, gain=0.5, y_scale=1 / 128)
+            tid = c.terrain(t)
             r = V.simulate(c, seed=3, rate=rate, duration=duration, speed=20.0, radius=16,
                            classes=((16, 1.0, tid), (32, 1.0, tid), (64, 1.0, tid)))
-            out.append({"rate_req_s": rate, "requests": r.requests, "completed": r.completed,
+            out.append({"rate_req_s": rate, "budget_GB": budget_gb,
+                        "requests": r.requests, "completed": r.completed,
                         "cache_hits": r.cache_hits, "generated": r.generated,
                         "batches": r.batches, "evictions": r.evictions,
                         "throughput_req_s": round(r.throughput_rps, 1),
@@ -350,8 +394,9 @@ def serving_leg(rates=(2000, 5000, 10000, 20000), duration=1.5):
                         "miss_p50_ms": round(r.miss_p50_ms, 4),
                         "miss_p99_ms": round(r.miss_p99_ms, 4),
                         "peak_resident_GB": round(r.peak_resident_bytes / 2**30, 3)})
-            log(f"[bench] serving {rate} req/s: p50 {r.p50_ms:.3f} ms, p99 {r.p99_ms:.3f} ms "
-                f"(misses p99 {r.miss_p99_ms:.3f} ms), {r.generated} generated")
+            log(f"[bench] serving {rate} req/s, {budget_gb} GB: p50 {r.p50_ms:.3f} ms, p99 "
+                f"{r.p99_ms:.3f} ms (misses p99 {r.miss_p99_ms:.3f} ms), {r.generated} generated, "
+                f"{r.evictions} evicted")
     return out
 
 
@@ -368,9 +413,12 @@ def main():
     ap.add_argument("--no-world", action="store_true", help="skip the tick-by-tick world leg")
     ap.add_argument("--no-two", action="store_true",
                     help="skip the two-batches-in-flight leg (two streams)")
-    ap.add_argument("--pmc-run", choices=["fused", "staged"], default=None,
-                    help="counter-collection run: 2 runs of the batch in one form, then a "
-                         "256 MiB plain copy kernel (the byte-counter reference); no timing")
+    ap.add_argument("--pmc-run", choices=["fused", "staged", "instances", "config3", "frame",
+                                          "server"], default=None,
+                    help="profiler run of one workload, no timing: the config-2 batch in one "
+                         "form (vertices+indices, or InstanceRaw staged), the config-3 ragged "
+                         "batch, the config-5 frame, or server flights at 5000 req/s; then a "
+                         "256 MiB plain copy kernel (the byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -381,12 +429,26 @@ def main():
     ctx = V.Context(0)
     if args.pmc_run:
         t, coords, nb = workload(ctx)
-        b = V.Batch(ctx, coords, EDGE, ctx.terrain(t), nb, formats=V.FMT_VERTICES)
-        b.set_mode(V.BATCH_FUSED if args.pmc_run == "fused" else V.BATCH_STAGED)
-        b.run()
-        faces = b.wait()
-        b.run()
-        b.wait()
+        faces = None
+        if args.pmc_run in ("fused", "staged", "instances"):
+            fmt = V.FMT_INSTANCES if args.pmc_run == "instances" else V.FMT_VERTICES
+            b = V.Batch(ctx, coords, EDGE, ctx.terrain(t), nb, formats=fmt)
+            b.set_mode(V.BATCH_FUSED if args.pmc_run == "fused" else V.BATCH_STAGED)
+        elif args.pmc_run == "config3":
+            b, _ = config3_batch(ctx)
+            b.set_mode(V.BATCH_STAGED)
+        if args.pmc_run == "frame":
+            faces = frame_leg(ctx, 2746.6, frames=2)["new_faces_per_frame"]
+        elif args.pmc_run == "server":
+            tid = ctx.terrain(t)
+            r = V.simulate(ctx, seed=3, rate=5000, duration=0.3, speed=20.0, radius=16,
+                           classes=((16, 1.0, tid), (32, 1.0, tid), (64, 1.0, tid)))
+            faces = int(r.faces)
+        else:
+            b.run()
+            faces = b.wait()  # sizes the arena
+            b.run()
+            b.wait()
         nbytes = 256 << 20
         src, dst = V.DeviceBuffer(ctx, nbytes), V.DeviceBuffer(ctx, nbytes)
         for _ in range(2):
@@ -486,6 +548,10 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic("emit_kernel<32>", total_faces),
                 "ms": round(float(stage_ms[3]), 4)}
+    if roof.get("traffic") is not None:
+        roof["traffic_source"] = ("read from profiles/traffic_uniform4096x32.txt: rocprofv3 "
+                                  "FETCH_SIZE + WRITE_SIZE of this kernel on this workload, scaled "
+                                  "by a plain copy kernel's counters; not measured in this run")
     stages["fill"]["traffic"] = measured_traffic("fill_kernel<32>", total_faces)
     stages["emit"]["traffic"] = measured_traffic("emit_kernel<32>", total_faces)
 
@@ -504,6 +570,10 @@ def main():
                  "ms_per_step": round(pe * 1e3 / reps, 3),
                  "launches_per_step": 3 * n + 1}
         log(f"[bench] plain per-chunk path: {plain['value']:.0f} chunks/s")
+
+    inst = instances_leg(ctx, t, coords, nb, args.steps)
+    log(f"[bench] InstanceRaw output: {inst['ms_per_step']} ms/step, emit {inst['emit_ms']} ms "
+        f"({inst['emit_GB/s']} GB/s)")
 
     two = None
     if not args.no_two:
@@ -563,6 +633,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "two_batches_in_flight": two,
+        "instances_format": inst,
         "other_form_gpu": other,
         "full_frame": frame,
         "other_configs": configs,
@@ -570,11 +641,18 @@ def main():
         "world": world,
     }
     if serving:
-        # request-to-mesh latency of the streaming config at its highest rate
-        top = max(serving, key=lambda r: r["rate_req_s"])
+        # request-to-mesh latency of the streaming config (8 GB budget) at its
+        # highest rate, and the same rate with LRU eviction running
+        top = max((r for r in serving if r["budget_GB"] == 8.0), key=lambda r: r["rate_req_s"])
         out["latency_ms"] = {"rate_req_s": top["rate_req_s"], "p50": top["p50_ms"],
                              "p99": top["p99_ms"], "miss_p50": top["miss_p50_ms"],
                              "miss_p99": top["miss_p99_ms"]}
+        tight = [r for r in serving if r["budget_GB"] < 8.0]
+        if tight:
+            tt = min(tight, key=lambda r: r["budget_GB"])
+            out["latency_ms"]["under_pressure"] = {
+                "budget_GB": tt["budget_GB"], "evictions": tt["evictions"], "p50": tt["p50_ms"],
+                "p99": tt["p99_ms"], "miss_p99": tt["miss_p99_ms"]}
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)
     if plain:

@@ -84,6 +84,14 @@ typedef struct vx_coord {
  *   VX_TERRAIN_HEIGHTMAP  n = fbm2(wx*f, wz*f); h = floor(height_scale*(n+1)/2)
  *                         clamped to [0, height_max]; voxel solid iff wy <= h
  *   VX_TERRAIN_DENSITY    solid iff fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0
+ *   VX_TERRAIN_REFERENCE  solid iff fbm3(wx*f, wz*f, wy*f) < 0: the reference's
+ *                         own rule (noise < 0 -> GRASS, src/chunk.rs:38-42) and
+ *                         axis order (world z on the noise's y axis, world y
+ *                         on its z axis, src/perlin_noise.rs:6-13); y_scale
+ *                         unused.  The reference's parameters are seed 1,
+ *                         frequency 0.09, gain 2.5, lacunarity 0.5, 3 octaves
+ *                         (src/perlin_noise.rs:14-18), with Perlin's noise in
+ *                         place of simdnoise's.  Batches of it run staged.
  * fbm: sum over `octaves` of amp*perlin(p*freq), freq *= lacunarity,
  * amp *= gain, from freq = frequency, amp = 1 (the reference's fbm_3d_offset
  * with_freq/with_gain/with_lacunarity/with_octaves, src/perlin_noise.rs:6-20).
@@ -93,6 +101,7 @@ typedef struct vx_coord {
 #define VX_TERRAIN_SOLID 1
 #define VX_TERRAIN_HEIGHTMAP 2
 #define VX_TERRAIN_DENSITY 3
+#define VX_TERRAIN_REFERENCE 4
 
 typedef struct vx_terrain {
     int32_t kind;
@@ -387,10 +396,13 @@ int vx_server_pump(vx_server *server, int32_t wait_us);
 int vx_server_poll(vx_server *server, vx_completion *out, int32_t cap, int32_t *n);
 /* Blocks until every submitted request has completed. */
 int vx_server_drain(vx_server *server);
-/* Resident chunk + mesh for (coord, edge), retained for the caller (either
- * output may be NULL); VX_ERR_NOT_READY when not resident. */
-int vx_server_lookup(vx_server *server, vx_coord coord, int32_t edge, vx_chunk **chunk,
-                     vx_mesh **mesh);
+/* Resident chunk + mesh for (coord, edge, terrain_id), retained for the
+ * caller (either output may be NULL); VX_ERR_NOT_READY when not resident.
+ * The cache keys chunks by coordinate, edge and terrain: requests for one
+ * coordinate under two terrains are two chunks, and a neighbour snapshot
+ * (VX_NB_RESIDENT) only takes neighbours of the request's own terrain. */
+int vx_server_lookup(vx_server *server, vx_coord coord, int32_t edge, int32_t terrain_id,
+                     vx_chunk **chunk, vx_mesh **mesh);
 int vx_server_stats_get(vx_server *server, vx_server_stats *stats);
 
 /* Streaming simulation: a camera on a seeded random walk, Poisson request

@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_voxel.so")
 
 AIR, GRASS, DIRT = 0, 1, 2
 LEFT, RIGHT, BACK, FRONT, DOWN, UP = range(6)
-TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY = range(4)
+TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY, TERRAIN_REFERENCE = range(5)
 
 
 class Terrain(C.Structure):
@@ -161,7 +161,7 @@ def chunk_fill(t: Terrain, cx: int, cy: int, cz: int, edge: int, fast: bool = Fa
 
 
 def noise_block(t: Terrain, cx: int, cy: int, cz: int, edge: int) -> np.ndarray:
-    n = edge ** 3 if t.kind == TERRAIN_DENSITY else edge ** 2
+    n = edge ** 3 if t.kind in (TERRAIN_DENSITY, TERRAIN_REFERENCE) else edge ** 2
     out = np.zeros(n, np.float32)
     lib().vxo_noise_block(C.byref(t), cx, cy, cz, edge, _p(out))
     return out

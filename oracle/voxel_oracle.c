@@ -264,6 +264,10 @@ void vxo_fbm3_row8(const uint8_t *perm, int32_t x0, int32_t y, int32_t z, int oc
 /*   heightmap: n = fbm2(wx*f, wz*f); h = floor(height_scale*(n+1)/2),       */
 /*              clamped to [0, height_max]; solid iff wy <= h               */
 /*   density:   solid iff fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0            */
+/*   reference: solid iff fbm3(wx*f, wz*f, wy*f) < 0 -- the reference's own  */
+/*              rule (chunk.rs:38-42) and axis order: perlin_noise.rs:6-13   */
+/*              passes world z as the noise's y offset and world y as its z  */
+/*              offset, so noise[x + 32z + 1024y] is sampled at (x, z, y)    */
 /* with (wx, wy, wz) = chunk*edge + local (coordinate.rs:60-65 generalised). */
 /* ------------------------------------------------------------------------ */
 static inline int32_t heightmap_h(const vxo_terrain *t, const uint8_t *perm, int32_t wx,
@@ -304,6 +308,31 @@ static void chunk_fill_impl(const vxo_terrain *t, int32_t cx, int32_t cy, int32_
                     ids[(size_t)x + (size_t)N * z + NN * y] =
                         (oy + y) <= h[(size_t)z * N + x] ? VXO_GRASS : VXO_AIR;
         free(h);
+        return;
+    }
+    if (t->kind == VXO_TERRAIN_REFERENCE) {
+        for (int32_t y = 0; y < N; y++) {
+            const int32_t wy = oy + y;
+            for (int32_t z = 0; z < N; z++) {
+                const int32_t wz = oz + z;
+                int32_t x = 0;
+                if (fast) {
+                    for (; x + 8 <= N; x += 8) {
+                        float d[8];
+                        fbm3_row8_with(perm, p32, ox + x, wz, wy, t->octaves, t->frequency,
+                                       t->lacunarity, t->gain, d);
+                        for (int i = 0; i < 8; i++)
+                            ids[(size_t)(x + i) + (size_t)N * z + NN * y] =
+                                d[i] < 0.0f ? VXO_GRASS : VXO_AIR;
+                    }
+                }
+                for (; x < N; x++) {
+                    float d = vxo_fbm3(perm, (float)(ox + x), (float)wz, (float)wy, t->octaves,
+                                       t->frequency, t->lacunarity, t->gain);
+                    ids[(size_t)x + (size_t)N * z + NN * y] = d < 0.0f ? VXO_GRASS : VXO_AIR;
+                }
+            }
+        }
         return;
     }
     /* density */
@@ -355,12 +384,15 @@ void vxo_noise_block(const vxo_terrain *t, int32_t cx, int32_t cy, int32_t cz, i
                              t->lacunarity, t->gain);
         return;
     }
+    const int ref = t->kind == VXO_TERRAIN_REFERENCE; /* noise axes (x, z, y) */
     for (int32_t y = 0; y < N; y++)
         for (int32_t z = 0; z < N; z++)
-            for (int32_t x = 0; x < N; x++)
+            for (int32_t x = 0; x < N; x++) {
+                const float wx = (float)(ox + x), wy = (float)(oy + y), wz = (float)(oz + z);
                 out[(size_t)x + (size_t)N * z + (size_t)N * N * y] =
-                    vxo_fbm3(perm, (float)(ox + x), (float)(oy + y), (float)(oz + z), t->octaves,
-                             t->frequency, t->lacunarity, t->gain);
+                    vxo_fbm3(perm, wx, ref ? wz : wy, ref ? wy : wz, t->octaves, t->frequency,
+                             t->lacunarity, t->gain);
+            }
 }
 
 void vxo_occupancy(const uint8_t *ids, int32_t N, uint32_t *occ) {

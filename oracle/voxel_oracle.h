@@ -32,7 +32,12 @@
  *   vxo_perm                  splitmix64 + Fisher-Yates; pinned by
  *                             splitmix64's published first outputs.
  *   vxo_chunk_fill            config rules; pinned by closed forms (all-air,
- *                             all-solid, heightmap monotone columns).
+ *                             all-solid, heightmap monotone columns).  The
+ *                             reference rule (chunk.rs:38-42, noise axes of
+ *                             perlin_noise.rs:6-13) is pinned against the
+ *                             density rule: its noise block is the density
+ *                             noise block with y and z transposed, bit for
+ *                             bit, and a voxel is solid iff that value < 0.
  *   vxo_mesh_faces            chunk.rs:59-245; pinned by closed forms
  *                             (single voxel, all-solid with/without
  *                             neighbours, checkerboard) and an exhaustive
@@ -73,7 +78,8 @@ enum {
     VXO_TERRAIN_AIR = 0,       /* every voxel air */
     VXO_TERRAIN_SOLID = 1,     /* every voxel grass */
     VXO_TERRAIN_HEIGHTMAP = 2, /* 2-D fBm heightmap column rule */
-    VXO_TERRAIN_DENSITY = 3    /* 3-D fBm density rule */
+    VXO_TERRAIN_DENSITY = 3,   /* 3-D fBm density rule */
+    VXO_TERRAIN_REFERENCE = 4  /* the reference's rule: fbm3(x, z, y) < 0 */
 };
 
 typedef struct vxo_terrain {

@@ -34,6 +34,9 @@ TERRAINS = {
                             height_scale=0.0, height_max=0, y_scale=1 / 128),
     "reference_params": dict(kind=3, seed=1, octaves=3, frequency=0.09, lacunarity=0.5, gain=2.5,
                              height_scale=0.0, height_max=0, y_scale=0.0),
+    # the reference's own rule and axis order: grass iff fbm3(wx*f, wz*f, wy*f) < 0
+    "reference_rule": dict(kind=4, seed=1, octaves=3, frequency=0.09, lacunarity=0.5, gain=2.5,
+                           height_scale=0.0, height_max=0, y_scale=0.0),
 }
 
 
@@ -92,6 +95,9 @@ def main():
         if kw["kind"] == 3:
             vals = [O.fbm3(p, float(x), float(y), float(z), kw["octaves"], kw["frequency"],
                            kw["lacunarity"], kw["gain"]) for x, y, z in pts]
+        elif kw["kind"] == 4:  # noise sampled at (wx, wz, wy)
+            vals = [O.fbm3(p, float(x), float(z), float(y), kw["octaves"], kw["frequency"],
+                           kw["lacunarity"], kw["gain"]) for x, y, z in pts]
         else:
             vals = [O.fbm2(p, float(x), float(z), kw["octaves"], kw["frequency"], kw["lacunarity"],
                            kw["gain"]) for x, _, z in pts]
@@ -119,8 +125,14 @@ def main():
     nb5 = np.array([[-1, 1, -1, -1], [0, -1, -1, -1]], np.int32)
     out["batches"]["config5_pair_e64"] = dict(terrain="config5_density",
                                               **batch_digest(t5, c5, [64] * 2, nb5))
+    # the reference's own rule, 3x3 grid of 32^3 with in-grid neighbours
+    tr = O.terrain(**TERRAINS["reference_rule"])
+    cr = [(x - 1, 0, z - 1) for x in range(3) for z in range(3)]
+    out["batches"]["reference_rule_grid3x3_e32"] = dict(terrain="reference_rule",
+                                                        **batch_digest(tr, cr, [32] * 9, grid_nb(3, 3)))
     # frustum: seeded camera, 64x64 chunk grid at y = 0 and y = -1
-    import test_oracle as T
+    sys.path.insert(0, os.path.join(ROOT, "wgpu-voxel-engine_amd"))
+    import camera as T
     eye = np.array([1000.0, 40.0, 1000.0])
     m = T.colmajor(T.perspective(np.radians(70), 16 / 9, 0.1, 1024) @
                    T.look_at(eye, eye + np.array([0.6, -0.1, 0.8])))

@@ -6,6 +6,9 @@
 // masks and draw lists -- so that AddressSanitizer and UBSan, compiled into
 // the host code only, see its allocations, reference counts and copies.
 // The kernels run normally.  Exit status 0 and "host check ok" on success.
+#include <sanitizer/lsan_interface.h>
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -203,7 +206,7 @@ static int server(int32_t tid_seed) {
     EXPECT(st.evictions > 0);
     vx_chunk *c = nullptr;
     vx_mesh *m = nullptr;
-    CHECK(vx_server_lookup(s, reqs.back().coord, reqs.back().edge, &c, &m));
+    CHECK(vx_server_lookup(s, reqs.back().coord, reqs.back().edge, reqs.back().terrain_id, &c, &m));
     // a 64^3 request and 120 more queued, left in flight at destruction
     vx_request big{};
     big.coord = vx_coord{100, 0, 100};
@@ -286,7 +289,20 @@ int main() {
     std::fflush(stdout);
     CHECK(vx_ctx_destroy(ctx));
     if (server(5)) return 1;
-    std::printf("server ok\nhost check ok\n");
-    std::fflush(stdout);  // a leak report at exit would end the process unflushed
-    return 0;
+    std::printf("server ok\n");
+    std::fflush(stdout);
+    // Leak check now, with every library object released, then leave without
+    // running the HIP runtime's own static teardown: the sanitizer runtime's
+    // device-memory bookkeeping cannot release blocks once that teardown has
+    // unloaded the HSA runtime, which is no property of this library.
+    const int leaks = __lsan_do_recoverable_leak_check();
+    if (leaks) {
+        std::fprintf(stderr, "leak check failed\n");
+        std::fflush(stderr);
+        _exit(1);
+    }
+    std::printf("host check ok\n");
+    std::fflush(stdout);
+    std::fflush(stderr);
+    _exit(0);
 }

@@ -63,7 +63,7 @@ def test_server_request_beyond_budget_fails_with_status(oracle_mod):
             comps = {cm.ticket: cm for cm in s.poll()}
             assert all(comps[t].status == V.OK for t in tickets)
             assert s.stats().peak_resident_bytes <= budget
-            chunk, mesh = s.lookup((2, 0, 0), 16)
+            chunk, mesh = s.lookup((2, 0, 0), 16, tid)
             ids = oracle_mod.chunk_fill(oterrain(oracle_mod, C2), 2, 0, 0, 16)
             assert (chunk.ids() == ids).all()
             assert_mesh_equal(oracle_mod, mesh, ids, 16, (2, 0, 0), (None,) * 4)
@@ -100,14 +100,14 @@ def test_handles_outlive_eviction_and_batch(oracle_mod):
             s.submit([((0, 0, 0), 32, tid, V.NB_NONE)])
             s.drain()
             s.poll()
-            chunk, mesh = s.lookup((0, 0, 0), 32)
+            chunk, mesh = s.lookup((0, 0, 0), 32, tid)
             for k in range(8):  # enough 32^3 requests to cycle the cache several times
                 s.submit([((x, 0, 10 + k), 32, tid, V.NB_NONE) for x in range(40)])
                 s.drain()
                 s.poll()
             st = s.stats()
             assert st.evictions > 0
-            assert s.lookup((0, 0, 0), 32) is None  # evicted from the cache
+            assert s.lookup((0, 0, 0), 32, tid) is None  # evicted from the cache
             ids = oracle_mod.chunk_fill(t, 0, 0, 0, 32)
             assert (chunk.ids() == ids).all()
             assert_mesh_equal(oracle_mod, mesh, ids, 32, (0, 0, 0), (None,) * 4)
@@ -121,3 +121,52 @@ def test_handles_outlive_eviction_and_batch(oracle_mod):
         b.close()
         assert (ch.ids() == oracle_mod.chunk_fill(t, 5, 0, 6, 32)).all()
         ch.release()
+
+
+def test_arena_growth_refused_then_recovers(oracle_mod):
+    """A batch created with room for 16 faces overflows its mesh arena; the
+    grown arena is refused by the budget (a ballast buffer holds the rest),
+    so vx_batch_wait returns VX_ERR_BUDGET and the batch holds no readable
+    mesh.  A run with the ballast still held is refused the same way; once
+    the ballast is freed the next run grows the arena, re-captures its graph
+    and matches the oracle byte for byte."""
+    say("arena growth refused by the budget")
+    budget = 48 << 20
+    coords = [(x, 0, z) for x in range(4) for z in range(4)]
+    nb = V.grid_neighbours(4, 4)
+    with V.Context(0, hbm_budget=budget) as c:
+        tid = c.terrain(C2)
+        b = V.Batch(c, coords, 32, tid, nb, formats=V.FMT_VERTICES, face_capacity=16)
+        # 16 chunks of this terrain hold about 44,000 faces (4.6 MB of
+        # vertices and indices): hold all but 1-2 MiB of the budget in
+        # 1 MiB blocks (a size class of the heap, so nothing is rounded up)
+        ballast = []
+        while budget - c.memory()[0] > (2 << 20):
+            ballast.append(V.DeviceBuffer(c, 1 << 20))
+        for attempt in range(2):
+            b.run()
+            with pytest.raises(V.VoxelError) as e:
+                b.wait()
+            assert e.value.status == V.ERR_BUDGET, attempt
+            with pytest.raises(V.VoxelError) as e:
+                b.chunk_faces(0)
+            assert e.value.status == V.ERR_NOT_READY
+        for blk in ballast:
+            blk.free()
+        b.run()
+        total = b.wait()
+        assert total > 16
+        t = oterrain(oracle_mod, C2)
+        ids = [oracle_mod.chunk_fill(t, *cc, 32) for cc in coords]
+        verts, idxs = b.mesh(V.FMT_VERTICES), b.mesh(V.FMT_INDICES)
+        run = 0
+        for i, cc in enumerate(coords):
+            adj = [ids[j] if j >= 0 else None for j in nb[i]]
+            faces, _, vert, idx = oracle_mod.mesh_buffers(ids[i], 32, tuple(v * 32 for v in cc), adj)
+            first, cnt = b.chunk_faces(i)
+            assert (first, cnt) == (run, len(faces))
+            assert (verts[4 * first:4 * (first + cnt)].view(np.uint32) == vert.view(np.uint32)).all()
+            assert (idxs[6 * first:6 * (first + cnt)] == idx).all()
+            run += cnt
+        assert run == total
+        b.close()

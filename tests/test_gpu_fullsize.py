@@ -126,7 +126,7 @@ def test_config5_full_resident_grid_and_cull(ctx, oracle_mod):
     past every 32-bit byte offset, and 4.4 GB of indices) byte for byte
     against the oracle driver's meshes at the same bases, compared in slices;
     and the cull mask of every AABB for three cameras."""
-    import test_oracle as T
+    import camera as T
     O = oracle_mod
     say("config 5 full")
     side = 256

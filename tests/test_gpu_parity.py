@@ -2,6 +2,7 @@
 compared with the CPU oracle on the same seeded inputs -- bit for bit
 (block ids, occupancy, noise values, vertex/index buffers, InstanceRaw
 records, cull masks)."""
+import itertools
 import sys
 
 import numpy as np
@@ -18,6 +19,12 @@ C2 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacu
 C5 = V.Terrain(kind=V.TERRAIN_DENSITY, seed=5, octaves=4, frequency=1 / 64, y_scale=1 / 128)
 REF_NOISE = V.Terrain(kind=V.TERRAIN_DENSITY, seed=1, octaves=3, frequency=0.09, lacunarity=0.5,
                       gain=2.5, y_scale=0.0)  # the reference's own fbm parameters, threshold 0
+# the reference's own rule and axis order (noise at (wx, wz, wy) < 0 -> grass,
+# src/perlin_noise.rs:6-20, src/chunk.rs:38-42) with its parameters
+REF_RULE = V.Terrain(kind=V.TERRAIN_REFERENCE, seed=1, octaves=3, frequency=0.09, lacunarity=0.5,
+                     gain=2.5)
+REF_RULE5 = V.Terrain(kind=V.TERRAIN_REFERENCE, seed=9, octaves=5, frequency=1 / 40,
+                      lacunarity=2.0, gain=0.5)  # > 4 octaves: the octave-outer loop
 # octave counts beyond the configs' 1, 3 and 4: 2 runs the octave-inner row
 # loop, 5 and 8 (the maximum) the octave-outer loop; non-power-of-two gain
 # and lacunarity
@@ -78,6 +85,8 @@ def assert_mesh_equal(O, mesh, ids, edge, coord, adj_ids):
     (OCT8, (2, 0, 2), 32), (OCT8, (-1, 1, -9), 16), (HMAP8, (7, 0, -3), 32),
     (BIGF, (2 ** 26 - 2, 0, 1 - 2 ** 26), 32), (BIGF, (1 - 2 ** 26, 3, 2 ** 26 - 2), 32),
     (HMAPBIG, (2 ** 26 - 2, 0, 2 ** 26 - 2), 32),
+    (REF_RULE, (0, 0, 0), 32), (REF_RULE, (-2, 1, 3), 16), (REF_RULE, (1, 0, -1), 64),
+    (REF_RULE5, (3, -1, 2), 32),
 ])
 def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
     say("noise", terrain.kind, terrain.seed, coord, edge)
@@ -93,6 +102,8 @@ def test_noise_bit_exact(ctx, oracle_mod, terrain, coord, edge):
     (C2, (1, 0, 1), 64), (REF_NOISE, (0, 0, 0), 32),
     (V.Terrain(kind=V.TERRAIN_AIR), (0, 0, 0), 32), (V.Terrain(kind=V.TERRAIN_SOLID), (2, 0, 1), 64),
     (OCT2, (3, 0, 3), 32), (OCT5, (1, 0, -2), 32), (OCT8, (0, 0, 0), 64), (HMAP8, (-2, 0, 4), 16),
+    (REF_RULE, (0, 0, 0), 32), (REF_RULE, (3, -1, 2), 64), (REF_RULE, (-5, 0, 4), 16),
+    (REF_RULE5, (0, 0, 1), 32),
 ])
 def test_fill_ids_and_occupancy(ctx, oracle_mod, terrain, coord, edge):
     say("fill", terrain.kind, terrain.seed, coord, edge)
@@ -162,6 +173,36 @@ def test_single_voxels_at_every_corner(ctx, oracle_mod):
                 ids[x + edge * z + edge * edge * y] = 1
                 ch = V.Chunk.from_ids(ctx, (-1, 0, 1), edge, ids)
                 assert_mesh_equal(oracle_mod, ch.create_mesh(), ids, edge, (-1, 0, 1), (None,) * 4)
+
+
+def test_faces_near_world_zero_bit_exact(ctx, oracle_mod):
+    """Single voxels at world coordinates -1, 0 and +1 on every axis, each
+    alone in a 16^3 chunk: InstanceRaw and vertex records bit-exact against
+    the oracle and, for the non-zero matrix entries and every vertex
+    position, against the independent textbook evaluation of quad.rs:14-54
+    and vertex.vert:24 (the ~4.4e-8 rotation terms set the low bits here)."""
+    from test_oracle import expected_vertices
+    edge = 16
+    for p in itertools.product((-1, 0, 1), repeat=3):
+        coord = tuple(v // edge for v in p)
+        local = tuple(v - c * edge for v, c in zip(p, coord))
+        ids = np.zeros(edge ** 3, np.uint8)
+        ids[local[0] + edge * local[2] + edge * edge * local[1]] = 1
+        say("face bits near zero", p)
+        mesh = V.Chunk.from_ids(ctx, coord, edge, ids).create_mesh()
+        assert_mesh_equal(oracle_mod, mesh, ids, edge, coord, (None,) * 4)
+        inst, vert = mesh.instances(), mesh.vertices()
+        assert mesh.faces == 6
+        for d in range(6):  # one voxel: faces in direction order L R B F D U
+            R, t, ev = expected_vertices(d, p)
+            E = np.zeros((4, 4), np.float32)
+            E[:3, :3] = R.T
+            E[3, :3] = t
+            E[3, 3] = 1
+            m = inst[d]["m"]
+            nz = E != 0
+            assert (bits(m[nz]) == bits(E[nz])).all() and (m[~nz] == 0).all(), (p, d)
+            assert (bits(vert[4 * d:4 * d + 4]["pos"]) == bits(ev)).all(), (p, d)
 
 
 def test_generated_chunk_mesh_with_generated_neighbours(ctx, oracle_mod):
@@ -279,13 +320,35 @@ def test_batch_rows_split_over_workgroups(ctx, oracle_mod, n):
     say("batch split fill", n)
     rng = np.random.default_rng(n)
     one = V.Terrain(kind=V.TERRAIN_DENSITY, seed=13, octaves=1, frequency=1 / 24, y_scale=1 / 64)
-    pool = [one, OCT2, REF_NOISE, C2, OCT5, C1]
+    pool = [one, OCT2, REF_NOISE, C2, OCT5, C1, REF_RULE]
     edges = [(16, 16, 16, 16, 16, 16, 32, 32, 64)[i % 9] for i in range(n)]
     terrains = [pool[int(k)] for k in rng.integers(0, len(pool), n)]
     cells = rng.choice(80 * 4 * 80, n, replace=False)
     coords = [(int(c // 320) - 40, int(c // 80 % 4) - 2, int(c % 80) - 40) for c in cells]
     b = batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1, np.int32),
                         formats=V.FMT_VERTICES, mode=V.BATCH_STAGED)
+    b.close()
+
+
+@pytest.mark.parametrize("shape", [pytest.param((4, 4, 32), id="4x4x32"),
+                                   pytest.param((32, 32, 16), id="32x32x16"),
+                                   pytest.param((3, 3, 64), id="3x3x64")])
+def test_batch_reference_rule(ctx, oracle_mod, shape):
+    """Grids in the reference's own terrain rule (noise at (wx, wz, wy) < 0,
+    its seed 1 / f 0.09 / gain 2.5 / lacunarity 0.5 / 3 octaves) with in-grid
+    neighbours: 16 chunks (rows split over 4 fill workgroups), 1024 chunks
+    (one fill workgroup each) and 64^3 chunks (8 parts); ids and meshes byte
+    for byte.  The fused form is refused for this terrain."""
+    gx, gz, edge = shape
+    say("batch reference rule", shape)
+    coords = [(x - gx // 2, 0, z - 1) for x in range(gx) for z in range(gz)]
+    nb = V.grid_neighbours(gx, gz)
+    b = batch_vs_oracle(ctx, oracle_mod, coords, [edge] * len(coords), [REF_RULE] * len(coords),
+                        nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES)
+    assert b.mode == V.BATCH_STAGED
+    with pytest.raises(V.VoxelError) as e:
+        b.set_mode(V.BATCH_FUSED)
+    assert e.value.status == V.ERR_INVALID
     b.close()
 
 
@@ -401,7 +464,7 @@ def test_same_alone_and_in_the_mix(ctx, oracle_mod):
 
 # ------------------------------------------------------------------ frustum
 def test_frustum_mask_matches_oracle(ctx, oracle_mod):
-    import test_oracle as T
+    import camera as T
     rng = np.random.default_rng(12)
     for _ in range(5):
         eye = rng.uniform(-500, 500, 3)
@@ -422,7 +485,7 @@ def test_frame_new_chunks_beside_resident_grid_with_cull(ctx, oracle_mod, mode):
     new chunks in the column x = 8 (their -x neighbours are resident chunks,
     +-z neighbours are in the same batch) and culls the resident AABBs in
     the same run."""
-    import test_oracle as T
+    import camera as T
     say("frame", mode)
     t = oterrain(oracle_mod, C5)
     tid = ctx.terrain(C5)
@@ -486,7 +549,7 @@ def oracle_draws(O, planes, coords, faces, first, edge, mode):
 @pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 70000])
 @pytest.mark.parametrize("mode", [V.DRAW_INSTANCED, V.DRAW_EXPANDED])
 def test_draw_list_matches_oracle(ctx, oracle_mod, n, mode):
-    import test_oracle as T
+    import camera as T
     say("draw list", n, mode)
     rng = np.random.default_rng(n + 7 * mode)
     eye = rng.uniform(-300, 300, 3)
@@ -515,7 +578,7 @@ def test_draw_list_reports_offsets_beyond_32_bits(ctx):
 def test_batch_draw_list(ctx, oracle_mod, mode):
     """A resident grid's culled draw list references exactly the visible
     chunks' slices of the batch arena."""
-    import test_oracle as T
+    import camera as T
     coords = [(x, 0, z) for x in range(8) for z in range(8)]
     b = V.Batch(ctx, coords, 32, ctx.terrain(C5), V.grid_neighbours(8, 8),
                 formats=V.FMT_VERTICES | V.FMT_INSTANCES)

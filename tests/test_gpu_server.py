@@ -20,9 +20,9 @@ def say(*a):
     sys.stdout.flush()
 
 
-def mesh_of(server, coord, edge):
-    got = server.lookup(coord, edge)
-    assert got is not None, f"{coord}/{edge} not resident"
+def mesh_of(server, coord, edge, tid):
+    got = server.lookup(coord, edge, tid)
+    assert got is not None, f"{coord}/{edge}/terrain {tid} not resident"
     return got
 
 
@@ -45,7 +45,7 @@ def test_server_results_bit_exact(ctx, oracle_mod):
             if (c, e) in seen:
                 continue
             seen.add((c, e))
-            chunk, mesh = mesh_of(s, c, e)
+            chunk, mesh = mesh_of(s, c, e, tid)
             ids = oracle_mod.chunk_fill(oterrain(oracle_mod, C2), *c, e)
             assert (chunk.ids() == ids).all()
             assert comp.faces == mesh.faces
@@ -61,7 +61,7 @@ def test_same_alone_and_in_the_mix(ctx, oracle_mod):
     with V.Server(ctx) as alone:
         alone.submit([(target[0], 32, tids[32], V.NB_NONE)])
         alone.drain()
-        _, m_alone = mesh_of(alone, *target)
+        _, m_alone = mesh_of(alone, *target, tids[32])
         v_alone, i_alone = m_alone.vertices(), m_alone.indices()
     rng = np.random.default_rng(7)
     mix = [((int(x), 0, int(z)), int(e), tids[int(e)], V.NB_NONE)
@@ -72,7 +72,7 @@ def test_same_alone_and_in_the_mix(ctx, oracle_mod):
             s.submit(mix[k:k + 37])
             s.pump(50)
         s.drain()
-        _, m_mix = mesh_of(s, *target)
+        _, m_mix = mesh_of(s, *target, tids[32])
         assert (m_mix.vertices().view(np.uint32) == v_alone.view(np.uint32)).all()
         assert (m_mix.indices() == i_alone).all()
         st = s.stats()
@@ -93,10 +93,54 @@ def test_resident_neighbour_snapshot(ctx, oracle_mod):
         s.drain()
         (comp,) = s.poll()
         assert list(comp.nb_present) == [0, 0, 1, 0]  # only -z (10,0,10) was resident
-        _, mesh = mesh_of(s, (10, 0, 11), 32)
+        _, mesh = mesh_of(s, (10, 0, 11), 32, tid)
         ids = oracle_mod.chunk_fill(t, 10, 0, 11, 32)
         adj = (None, None, oracle_mod.chunk_fill(t, 10, 0, 10, 32), None)
         assert_mesh_equal(oracle_mod, mesh, ids, 32, (10, 0, 11), adj)
+
+
+def test_two_terrains_same_coordinate(ctx, oracle_mod):
+    """One coordinate and edge under two terrains is two chunks: neither is
+    served from the other's cache entry or flight, and a resident-neighbour
+    snapshot only takes neighbours of the request's own terrain."""
+    say("server two terrains at one coordinate")
+    hm = HMAP[32]
+    ta, tb = ctx.terrain(C2), ctx.terrain(hm)
+    oa, ob = oterrain(oracle_mod, C2), oterrain(oracle_mod, hm)
+    with V.Server(ctx, max_batch=8) as s:
+        # same flight: the two requests must not coalesce
+        s.submit([((3, 0, 4), 32, ta, V.NB_NONE), ((3, 0, 4), 32, tb, V.NB_NONE)])
+        s.drain()
+        assert s.stats().coalesced == 0 and s.stats().generated == 2
+        # later requests: each is a cache hit on its own terrain's entry
+        s.submit([((3, 0, 4), 32, tb, V.NB_NONE), ((3, 0, 4), 32, ta, V.NB_NONE)])
+        s.pump(0)
+        s.drain()
+        # (4,0,4) is resident under terrain a only: a terrain-b request next
+        # to it must not take it as its -z neighbour
+        s.submit([((4, 0, 4), 32, ta, V.NB_NONE)])
+        s.drain()
+        s.poll()
+        s.submit([((3, 0, 5), 32, tb, V.NB_RESIDENT), ((4, 0, 5), 32, tb, V.NB_RESIDENT)])
+        s.drain()
+        comps = {(tuple(c.coord), c.edge): c for c in s.poll()}
+        assert list(comps[((3, 0, 5), 32)].nb_present) == [0, 0, 1, 0]  # b's (3,0,4)
+        assert list(comps[((4, 0, 5), 32)].nb_present) == [0, 0, 0, 0]  # a's (4,0,4) is not b's
+        for tid, ot in ((ta, oa), (tb, ob)):
+            chunk, mesh = mesh_of(s, (3, 0, 4), 32, tid)
+            ids = oracle_mod.chunk_fill(ot, 3, 0, 4, 32)
+            assert (chunk.ids() == ids).all(), f"terrain {tid}"
+            assert_mesh_equal(oracle_mod, mesh, ids, 32, (3, 0, 4), (None,) * 4)
+        _, mesh = mesh_of(s, (3, 0, 5), 32, tb)
+        adj = (None, None, oracle_mod.chunk_fill(ob, 3, 0, 4, 32), None)
+        assert_mesh_equal(oracle_mod, mesh, oracle_mod.chunk_fill(ob, 3, 0, 5, 32), 32,
+                          (3, 0, 5), adj)
+        _, mesh = mesh_of(s, (4, 0, 5), 32, tb)
+        assert_mesh_equal(oracle_mod, mesh, oracle_mod.chunk_fill(ob, 4, 0, 5, 32), 32,
+                          (4, 0, 5), (None,) * 4)
+        assert s.lookup((4, 0, 4), 32, tb) is None
+        st = s.stats()
+        assert st.cache_hits == 2 and st.failed == 0
 
 
 def test_cache_hits_and_coalescing(ctx):
@@ -133,7 +177,7 @@ def test_lru_eviction_under_budget(oracle_mod):
             assert st.evictions > 0
             assert st.peak_resident_bytes <= budget
             # the most recent chunk is still resident and correct
-            chunk, mesh = s.lookup((19, 0, 19), 32)
+            chunk, mesh = s.lookup((19, 0, 19), 32, tid)
             ids = oracle_mod.chunk_fill(oterrain(oracle_mod, C2), 19, 0, 19, 32)
             assert (chunk.ids() == ids).all()
             assert_mesh_equal(oracle_mod, mesh, ids, 32, (19, 0, 19), (None,) * 4)
@@ -147,3 +191,76 @@ def test_streaming_simulation_smoke(ctx):
     assert r.completed == r.requests > 1000
     assert r.generated > 0 and r.p99_ms > 0
     assert r.p50_ms <= r.p99_ms <= r.max_ms
+
+
+def test_streaming_under_binding_budget(oracle_mod):
+    """Continuous batching at 20,000 req/s for 0.4 s (a camera walk, mixed
+    16/32/64 sizes) under a 128 MiB budget that the working set (about
+    0.6 GB) overflows, so LRU eviction runs throughout.  Every request
+    completes OK; the meshes of a sample of misses, looked up the moment
+    they complete, match the oracle; one request's mesh equals its mesh when
+    served alone; the budget is never exceeded."""
+    import time
+    say("server under a binding budget")
+    budget = 128 << 20
+    rate, duration = 20000.0, 0.4
+    rng = np.random.default_rng(3)
+    with V.Context(0, hbm_budget=budget) as c:
+        tids = {e: c.terrain(HMAP[e]) for e in (16, 32, 64)}
+        target = ((3, 0, -2), 32)
+        with V.Server(c, max_batch=64, max_inflight=4) as alone:
+            alone.submit([(target[0], 32, tids[32], V.NB_NONE)])
+            alone.drain()
+            _, m = mesh_of(alone, *target, tids[32])
+            v_alone, i_alone = m.vertices(), m.indices()
+            m.release()
+        checked, target_seen = 0, False
+        with V.Server(c, max_batch=64, max_inflight=4) as s:
+            t0, sent, cx, cz, heading = time.perf_counter(), 0, 0.0, 0.0, 0.0
+            meta, target_ticket = {}, None
+            while True:
+                el = time.perf_counter() - t0
+                due = min(int(el * rate), int(duration * rate)) - sent
+                if due > 0:
+                    heading += rng.normal(0, 0.05)
+                    cx, cz = cx + 0.5 * np.cos(heading), cz + 0.5 * np.sin(heading)
+                    reqs = []
+                    for _ in range(due):
+                        r, a = 16 * 32 * np.sqrt(rng.random()), 2 * np.pi * rng.random()
+                        e = int(rng.choice([16, 32, 64]))
+                        coord = (int(np.floor((cx + r * np.cos(a)) / e)), 0,
+                                 int(np.floor((cz + r * np.sin(a)) / e)))
+                        reqs.append((coord, e, tids[e], V.NB_NONE))
+                    if target_ticket is None and sent > rate * duration / 2:
+                        reqs.append((target[0], 32, tids[32], V.NB_NONE))
+                    tickets = s.submit(reqs)
+                    for tk, rq in zip(tickets, reqs):
+                        meta[tk] = rq
+                    if target_ticket is None and sent > rate * duration / 2:
+                        target_ticket = tickets[-1]
+                    sent += due
+                s.pump(0)
+                for comp in s.poll():
+                    assert comp.status == V.OK, comp.status
+                    coord, e, tid, _ = meta[comp.ticket]
+                    if comp.ticket == target_ticket:
+                        _, m = mesh_of(s, coord, e, tid)  # looked up before any further pump
+                        assert (m.vertices().view(np.uint32) == v_alone.view(np.uint32)).all()
+                        assert (m.indices() == i_alone).all()
+                        m.release()
+                        target_seen = True
+                    elif not comp.cache_hit and checked < 8 and comp.ticket % 97 == 0:
+                        ch, m = mesh_of(s, coord, e, tid)
+                        ids = oracle_mod.chunk_fill(oterrain(oracle_mod, HMAP[e]), *coord, e)
+                        assert (ch.ids() == ids).all()
+                        assert_mesh_equal(oracle_mod, m, ids, e, coord, (None,) * 4)
+                        ch.release()
+                        m.release()
+                        checked += 1
+                if sent >= int(duration * rate) and s.stats().completed == s.stats().submitted:
+                    break
+            st = s.stats()
+            assert st.evictions > 0, "the budget did not bind"
+            assert st.failed == 0 and st.completed == st.submitted == sent + 1
+            assert st.peak_resident_bytes <= budget
+        assert target_seen and checked >= 3

@@ -16,7 +16,7 @@ def bits(a):
 
 
 def camera(pos, look):
-    import test_oracle as T
+    import camera as T
     pos = np.asarray(pos, np.float64)
     return T.colmajor(T.perspective(np.radians(70), 16 / 9, 0.1, 1024) @
                       T.look_at(pos, pos + np.asarray(look, np.float64)))

@@ -3,11 +3,13 @@ mathematics fix: published known answers, closed forms, invariants, an
 exhaustive brute force on tiny chunks and an independent NumPy
 implementation.  CPU only."""
 import itertools
+import math
 
 import numpy as np
 import pytest
 
 import np_noise
+from camera import colmajor, look_at, perspective
 
 # Ken Perlin's permutation from his reference ImprovedNoise implementation.
 PERLIN_P = [
@@ -181,6 +183,52 @@ def test_heightmap_columns_are_prefixes(oracle_mod):
     assert oracle_mod.chunk_fill(t, 0, 1, 0, 16).sum() == 0
 
 
+REF_PARAMS = dict(seed=1, octaves=3, frequency=0.09, lacunarity=0.5, gain=2.5)
+
+
+@pytest.mark.parametrize("params", [REF_PARAMS,
+                                    dict(seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
+                                         gain=0.5)])
+def test_reference_rule_noise_is_density_noise_transposed(oracle_mod, params):
+    """The reference rule samples the noise at (wx, wz, wy)
+    (perlin_noise.rs:6-13): its noise block at chunk (cx, cy, cz) is the
+    density rule's block at (cx, cz, cy) with y and z swapped, bit for bit,
+    and a voxel is grass iff that value is < 0 (chunk.rs:38-42)."""
+    O = oracle_mod
+    tr = O.terrain(kind=O.TERRAIN_REFERENCE, **params)
+    td = O.terrain(kind=O.TERRAIN_DENSITY, y_scale=0.0, **params)
+    for (cx, cy, cz), N in [((0, 0, 0), 16), ((-1, 2, 3), 16), ((5, -1, -7), 32)]:
+        nr = O.noise_block(tr, cx, cy, cz, N).reshape(N, N, N)           # [y][z][x]
+        nd = O.noise_block(td, cx, cz, cy, N).reshape(N, N, N)           # [z][y][x] here
+        assert (bits(nr) == bits(nd.transpose(1, 0, 2))).all()
+        ids = O.chunk_fill(tr, cx, cy, cz, N).reshape(N, N, N)
+        assert (ids == (nr < 0).astype(np.uint8)).all()
+        assert (O.chunk_fill(tr, cx, cy, cz, N, fast=True).reshape(N, N, N) == ids).all()
+        # and the complement of the density rule with threshold 0, except
+        # where the noise is exactly 0 (air under both rules)
+        dens = O.chunk_fill(td, cx, cz, cy, N).reshape(N, N, N).transpose(1, 0, 2)
+        assert ((ids == 1) == ((dens == 0) & (nr != 0))).all()
+    # the reference's parameters give a mixed chunk near the origin
+    ids0 = O.chunk_fill(tr, 0, 0, 0, 32)
+    assert 0 < ids0.sum() < ids0.size
+
+
+def test_reference_rule_noise_vs_numpy(oracle_mod):
+    """The reference rule's noise block against the independent NumPy
+    implementation evaluated at (wx, wz, wy)."""
+    O = oracle_mod
+    t = O.terrain(kind=O.TERRAIN_REFERENCE, **REF_PARAMS)
+    pn = np_noise.permutation(REF_PARAMS["seed"])
+    for c, N in [((2, -1, 4), 16), ((0, 0, 0), 32)]:
+        o = np.array(c) * N
+        y, z, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+        ref = np_noise.fbm3(pn, (x + o[0]).astype(np.float32), (z + o[2]).astype(np.float32),
+                            (y + o[1]).astype(np.float32), REF_PARAMS["octaves"],
+                            REF_PARAMS["frequency"], REF_PARAMS["lacunarity"],
+                            REF_PARAMS["gain"]).reshape(-1)
+        assert (bits(O.noise_block(t, *c, N)) == bits(ref)).all()
+
+
 def test_air_and_solid_kinds(oracle_mod):
     assert oracle_mod.chunk_fill(oracle_mod.terrain(kind=0), 0, 0, 0, 16).sum() == 0
     assert (oracle_mod.chunk_fill(oracle_mod.terrain(kind=1), 3, 1, 2, 16) == 1).all()
@@ -295,59 +343,155 @@ def test_rotations_orthonormal_and_face_normals(oracle_mod):
 
 
 def test_instance_translation_and_vertices(oracle_mod):
+    """Random faces: the translation column is the voxel, the texture index
+    passes through, every vertex equals the independent evaluation bit for
+    bit, sits on the face's side of the voxel, and the uv and index patterns
+    are quad.rs:66-89's."""
     rng = np.random.default_rng(6)
     pts = rng.integers(-5000, 5000, size=(40, 3))
-    faces = np.zeros(len(pts) * 6, oracle_mod.FACE_DTYPE)
-    for i, p in enumerate(pts):
-        for d in range(6):
-            f = faces[6 * i + d]
-            f["x"], f["y"], f["z"], f["dir"], f["tex"] = p[0], p[1], p[2], d, 7
+    faces = unit_faces(oracle_mod, pts)
+    faces["tex"] = 7
     inst, vert, idx = oracle_mod.expand(faces)
     assert (inst["tex"] == 7).all()
     for i, f in enumerate(faces):
         m = inst[i]["m"]  # [col][row]
         assert tuple(m[3][:3]) == (float(f["x"]), float(f["y"]), float(f["z"]))
         assert m[3][3] == 1.0 and (m[:3, 3] == 0).all()
-        n = np.array(NORMALS[int(f["dir"])], np.float64)
-        p = np.array([f["x"], f["y"], f["z"]], np.float64)
+        p = (int(f["x"]), int(f["y"]), int(f["z"]))
+        _, _, ev = expected_vertices(int(f["dir"]), p)
         corners = vert[4 * i:4 * i + 4]
-        for v in corners:
-            rel = v["pos"].astype(np.float64) - p
-            assert abs(rel @ n - 0.5) < 1e-6 * (1 + np.abs(p).max())
-            assert np.abs(np.abs(rel) - 0.5).max() < 1e-6 * (1 + np.abs(p).max())
+        assert (bits(corners["pos"]) == bits(ev)).all()
+        # the face lies on the voxel's side named by its direction
+        n = np.array(NORMALS[int(f["dir"])], np.float64)
+        rel = corners["pos"].astype(np.float64) - np.array(p, np.float64)
+        assert (rel @ n == 0.5).all()
         assert [tuple(v["uv"]) for v in corners] == [(0, 0), (1, 0), (0, 1), (1, 1)]
-        assert len({tuple(np.round(v["pos"].astype(np.float64) - p, 3)) for v in corners}) == 4
+        assert len({tuple(np.round(r, 3)) for r in rel}) == 4
     assert (idx.reshape(-1, 6) == (np.arange(len(faces))[:, None] * 4 + [0, 2, 1, 3, 1, 2])).all()
 
 
+# Exact bits of Quad::new's rotations (quad.rs:14-46).  Rust's
+# f32::to_radians is deg * (PI_f32 / 180_f32) in f32; sin and cos of that f32
+# angle, correctly rounded to f32, are computed here from Python's f64
+# math.sin / math.cos (each within 1 f64 ulp of the true value), after
+# checking that no f64 result lies within a few f64 ulps of an f32 rounding
+# midpoint (so the f32 rounding of the true value is the one taken).
+F32_DIRS = {5: ("roll", -90.0), 4: ("roll", 90.0), 0: ("pitch", -90.0), 1: ("pitch", 90.0),
+            3: ("none", 0.0), 2: ("pitch", 180.0)}  # UP DOWN LEFT RIGHT FRONT BACK
+
+
+def f32_correctly_rounded(v: float) -> np.float32:
+    r = np.float32(v)
+    if float(r) != v:
+        other = np.nextafter(r, np.float32(math.copysign(np.inf, v - float(r))))
+        mid = (float(r) + float(other)) / 2
+        assert abs(v - mid) > 8 * math.ulp(v), "too close to an f32 midpoint to decide"
+    return r
+
+
+def f32_sin_cos_deg(deg: float):
+    k = np.float32(np.float32(math.pi) / np.float32(180.0))
+    a = np.float32(np.float32(deg) * k)
+    return f32_correctly_rounded(math.sin(float(a))), f32_correctly_rounded(math.cos(float(a)))
+
+
+def textbook_rotation(d: int) -> np.ndarray:
+    """from_euler_angles(roll, pitch, 0) of a pure roll (about x) or pure
+    pitch (about y): Rx = [[1,0,0],[0,c,-s],[0,s,c]], Ry = [[c,0,s],[0,1,0],
+    [-s,0,c]], with the correctly rounded f32 (s, c).  Every product of the
+    general formula with the other angle's (sin, cos) = (0, 1) reduces to one
+    of these entries exactly, whatever its grouping; only the signs of the
+    zero entries depend on nalgebra's expression order."""
+    axis, deg = F32_DIRS[d]
+    s, c = f32_sin_cos_deg(deg)
+    one, zero = np.float32(1), np.float32(0)
+    if axis == "roll":
+        return np.array([[one, zero, zero], [zero, c, -s], [zero, s, c]], np.float32)
+    if axis == "pitch":
+        return np.array([[c, zero, s], [zero, one, zero], [-s, zero, c]], np.float32)
+    return np.eye(3, dtype=np.float32)
+
+
+def test_rotation_constants_are_the_f32_sin_cos():
+    """The closed-form values the rotation tables hold: sin/cos of the f32
+    angles of +-90 and 180 degrees."""
+    assert f32_sin_cos_deg(90.0) == (np.float32(1.0), np.float32(-4.371139e-08))
+    assert f32_sin_cos_deg(-90.0) == (np.float32(-1.0), np.float32(-4.371139e-08))
+    assert f32_sin_cos_deg(180.0) == (np.float32(-8.742278e-08), np.float32(-1.0))
+    assert bits(f32_sin_cos_deg(90.0)[1]) == 0xB33BBD2E
+
+
+def test_rotation_tables_bit_exact(oracle_mod):
+    """Every non-zero entry of the oracle's six rotations equals the
+    textbook matrix's bit for bit (no snapped cos(90) = 0, no transposed or
+    sign-flipped term); every zero entry is a zero of either sign."""
+    for d in range(6):
+        R = oracle_mod.rotation(d)
+        E = textbook_rotation(d)
+        nz = E != 0
+        assert (bits(R[nz]) == bits(E[nz])).all(), f"dir {d}"
+        assert (R[~nz] == 0).all(), f"dir {d}"
+        assert np.count_nonzero(R) == np.count_nonzero(E)
+
+
+def expected_vertices(d: int, p) -> tuple:
+    """InstanceRaw matrix (column-major [col][row]) and the four vertex
+    positions of a face of direction d at voxel p, evaluated independently
+    of the oracle: M = T * R with the textbook R, then vertex.vert:24's
+    instance * (corner, 1) one f32 operation at a time, left to right."""
+    R = textbook_rotation(d)
+    t = [np.float32(v) for v in p]
+    corners = [(-0.5, 0.5, 0.5), (0.5, 0.5, 0.5), (-0.5, -0.5, 0.5), (0.5, -0.5, 0.5)]
+    verts = []
+    for cx, cy, cz in corners:
+        pos = []
+        for r in range(3):
+            acc = R[r, 0] * np.float32(cx)
+            acc = acc + R[r, 1] * np.float32(cy)
+            acc = acc + R[r, 2] * np.float32(cz)
+            acc = acc + t[r] * np.float32(1)
+            pos.append(np.float32(acc))
+        verts.append(pos)
+    return R, t, np.array(verts, np.float32)
+
+
+def unit_faces(oracle_mod, pts):
+    faces = np.zeros(len(pts) * 6, oracle_mod.FACE_DTYPE)
+    for i, p in enumerate(pts):
+        for d in range(6):
+            f = faces[6 * i + d]
+            f["x"], f["y"], f["z"], f["dir"], f["tex"] = p[0], p[1], p[2], d, d
+    return faces
+
+
+def test_instances_and_vertices_bit_exact_near_zero(oracle_mod):
+    """Faces at world coordinates -1, 0 and +1 on every axis, in all six
+    directions: where the translation is 0 or +-1 the ~4.4e-8 rotation terms
+    reach the vertices' low bits (0.5 - 2.2e-8 rounds to 0.49999997).  The
+    oracle's InstanceRaw non-zero entries and vertex positions equal the
+    independent evaluation bit for bit; only zero signs are left open."""
+    pts = list(itertools.product((-1, 0, 1), repeat=3)) + [(1000, -77, 5), (-3, 2048, -9)]
+    faces = unit_faces(oracle_mod, pts)
+    inst, vert, _ = oracle_mod.expand(faces)
+    low_bits_seen = False
+    for i, f in enumerate(faces):
+        d = int(f["dir"])
+        R, t, ev = expected_vertices(d, (int(f["x"]), int(f["y"]), int(f["z"])))
+        m = inst[i]["m"]  # [col][row]
+        E = np.zeros((4, 4), np.float32)
+        E[:3, :3] = R.T
+        E[3, :3] = t
+        E[3, 3] = 1
+        nz = E != 0
+        assert (bits(m[nz]) == bits(E[nz])).all(), (i, d)
+        assert (m[~nz] == 0).all(), (i, d)
+        got = vert[4 * i:4 * i + 4]["pos"]
+        assert (bits(got) == bits(ev)).all(), (i, d, got, ev)
+        low_bits_seen |= bool((np.abs(ev - np.round(ev * 2) / 2) > 0).any())
+    assert low_bits_seen  # the cases do exercise the small rotation terms
+
+
 # ---------------------------------------------------------------- frustum
-def perspective(fovy, aspect, near, far):
-    f = 1.0 / np.tan(fovy / 2)
-    m = np.zeros((4, 4))
-    m[0, 0] = f / aspect
-    m[1, 1] = f
-    m[2, 2] = (far + near) / (near - far)
-    m[2, 3] = 2 * far * near / (near - far)
-    m[3, 2] = -1
-    return m
-
-
-def look_at(eye, target, up=(0, 1, 0)):
-    eye, target, up = map(lambda a: np.asarray(a, np.float64), (eye, target, up))
-    f = target - eye
-    f /= np.linalg.norm(f)
-    s = np.cross(f, up)
-    s /= np.linalg.norm(s)
-    u = np.cross(s, f)
-    m = np.eye(4)
-    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
-    m[:3, 3] = -m[:3, :3] @ eye
-    return m
-
-
-def colmajor(m):
-    return np.asarray(m, np.float32).T.reshape(16)
-
 
 def test_frustum_closed_forms(oracle_mod):
     vp = perspective(np.radians(70), 1.0, 0.1, 1024) @ look_at((0, 0, 0), (0, 0, -1))

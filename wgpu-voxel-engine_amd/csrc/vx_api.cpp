@@ -82,14 +82,18 @@ hipError_t DeviceHeap::alloc(size_t n, void **out) {
     return hipSuccess;
 }
 
-void DeviceHeap::release(void *p) {
+void DeviceHeap::release_pending(void *p) {
     if (!p) return;
     auto it = sizes_.find(p);
     if (it == sizes_.end()) return;
-    const size_t c = it->second;
-    live_ -= c;
-    cached_ += c;
-    free_[c].push_back(p);
+    live_ -= it->second;
+    cached_ += it->second;
+    pending_.push_back(p);
+}
+
+void DeviceHeap::flush_pending() {
+    for (void *p : pending_) free_[sizes_[p]].push_back(p);
+    pending_.clear();
 }
 
 void DeviceHeap::trim() {
@@ -129,30 +133,38 @@ void make_perm(uint32_t seed, int32_t perm[512]) {
     for (int i = 0; i < 512; i++) perm[i] = p[i & 255];
 }
 
-// Gradient coefficient table for Perlin's grad(hash, x, y, z): per corner k
-// (bit0 x+1, bit1 y+1, bit2 z+1) and hash h, (ax, az, gy) such that
-// fma(yr, gy, fma(ax, xr, az*zr)) == ((h&1)?-u:u) + ((h&2)?-v:v) exactly.
+// Gradient coefficient tables for Perlin's grad(hash, x, y, z) in the
+// column-hoisted evaluation: per corner k (bit0 x+1, bit1 row+1, bit2
+// col+1; row = world y, col = world z) and hash h, (a_x, a_col, g_row) such
+// that fma(row_r, g_row, fma(a_x, xr, a_col*col_r)) == ((h&1)?-u:u) +
+// ((h&2)?-v:v) exactly.  Table c samples the noise at (wx, wy, wz) (noise y
+// = row, noise z = col); table r at (wx, wz, wy), the reference's axis order
+// (src/perlin_noise.rs:6-13: noise y = col, noise z = row).
 void make_grad_lut(GradLut &L) {
     for (int k = 0; k < 8; k++) {
         // zero whose product with the corner's relative coordinate is -0:
         // the coordinate is >= 0 on the low side (x - X) and < 0 on the high
         // side (x - X - 1).
         const float zx = (k & 1) ? 0.0f : -0.0f;
-        const float zy = (k & 2) ? 0.0f : -0.0f;
-        const float zz = (k & 4) ? 0.0f : -0.0f;
+        const float zrow = (k & 2) ? 0.0f : -0.0f;
+        const float zcol = (k & 4) ? 0.0f : -0.0f;
         for (int h = 0; h < 16; h++) {
             const float su = (h & 1) ? -1.0f : 1.0f, sv = (h & 2) ? -1.0f : 1.0f;
-            float ax, az, gy;
-            if (h < 4) {  // u = x, v = y
-                ax = su; az = zz; gy = sv;
-            } else if (h < 8) {  // u = x, v = z
-                ax = su; az = sv; gy = zy;
-            } else if (h == 12 || h == 14) {  // u = y, v = x
-                gy = su; ax = sv; az = zz;
-            } else {  // u = y, v = z
-                gy = su; az = sv; ax = zx;
-            }
-            L.c[k][h] = make_float4(ax, az, gy, 0.0f);
+            // Perlin: u = h<8 ? x : y; v = h<4 ? y : (h==12||h==14 ? x : z)
+            // coefficient of the noise's x, y and z axes, zero elsewhere
+            float g[3];
+            const int ui = h < 8 ? 0 : 1;
+            const int vi = h < 4 ? 1 : (h == 12 || h == 14 ? 0 : 2);
+            g[0] = g[1] = g[2] = 0.0f;
+            g[ui] = su;
+            g[vi] = sv;
+            const bool nz[3] = {ui == 0 || vi == 0, ui == 1 || vi == 1, ui == 2 || vi == 2};
+            // table c: noise (x, y, z) = (x, row, col)
+            L.c[k][h] = make_float4(nz[0] ? g[0] : zx, nz[2] ? g[2] : zcol, nz[1] ? g[1] : zrow,
+                                    0.0f);
+            // table r: noise (x, y, z) = (x, col, row)
+            L.r[k][h] = make_float4(nz[0] ? g[0] : zx, nz[1] ? g[1] : zcol, nz[2] ? g[2] : zrow,
+                                    0.0f);
         }
     }
 }
@@ -248,6 +260,14 @@ void vx_ctx::set_error(hipError_t e, const char *what) {
 }
 
 int vx_ctx::dev_alloc(size_t n, void **out) {
+    if (heap.has_pending()) {
+        // blocks released since the last allocation become reusable once
+        // the context stream has run everything queued before their release
+        // (other streams of this library, the server's slots, only release
+        // blocks after their own work on them has completed)
+        if (free_fence) (void)hipEventSynchronize(free_fence);
+        heap.flush_pending();
+    }
     const size_t c = DeviceHeap::size_class(n);
     if (budget && heap.live() + c > budget) return VX_ERR_BUDGET;
     hipError_t e = heap.alloc(n, out);
@@ -258,7 +278,16 @@ int vx_ctx::dev_alloc(size_t n, void **out) {
     return VX_OK;
 }
 
-void vx_ctx::dev_free(void *p) { heap.release(p); }
+// A released block may still be read or written by work queued on the
+// context stream (an emit into a mesh whose handle the caller released right
+// away, a fill of a chunk released before it ran).  The fence recorded here
+// is waited on before the block is handed out again, so an allocation used
+// on another stream (a server slot) never overlaps that work.
+void vx_ctx::dev_free(void *p) {
+    if (!p) return;
+    if (free_fence) (void)hipEventRecord(free_fence, stream);
+    heap.release_pending(p);
+}
 
 int vx_ctx::ensure_scratch(size_t bytes) {
     if (scratch_bytes >= bytes) return VX_OK;
@@ -308,6 +337,11 @@ extern "C" int vx_ctx_create(const vx_ctx_desc *desc, vx_ctx **out) {
         delete c;
         return VX_ERR_DEVICE;
     }
+    if (hipEventCreateWithFlags(&c->free_fence, hipEventDisableTiming) != hipSuccess) {
+        c->free_fence = nullptr;
+        vx_ctx_destroy(c);
+        return VX_ERR_DEVICE;
+    }
     GradLut lut;
     make_grad_lut(lut);
     MeshTables tabs;
@@ -340,6 +374,7 @@ extern "C" int vx_ctx_destroy(vx_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->free_fence) (void)hipEventDestroy(ctx->free_fence);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;  // DeviceHeap frees every device block
     return VX_OK;
@@ -411,8 +446,9 @@ extern "C" int vx_buffer_copy(vx_ctx *ctx, void *dst, const void *src, uint64_t 
 
 extern "C" int vx_terrain_register(vx_ctx *ctx, const vx_terrain *t, int32_t *id) {
     if (!ctx || !t || !id) return VX_ERR_INVALID;
-    if (t->kind < VX_TERRAIN_AIR || t->kind > VX_TERRAIN_DENSITY) return VX_ERR_INVALID;
-    if (t->kind == VX_TERRAIN_HEIGHTMAP || t->kind == VX_TERRAIN_DENSITY) {
+    if (t->kind < VX_TERRAIN_AIR || t->kind > VX_TERRAIN_REFERENCE) return VX_ERR_INVALID;
+    if (t->kind == VX_TERRAIN_HEIGHTMAP || t->kind == VX_TERRAIN_DENSITY ||
+        t->kind == VX_TERRAIN_REFERENCE) {
         if (t->octaves < 1 || t->octaves > kMaxOctaves) return VX_ERR_INVALID;
         // every octave frequency must keep x - floor(x) - 1 strictly negative
         float f = t->frequency;
@@ -633,8 +669,8 @@ extern "C" int vx_chunk_noise(vx_ctx *ctx, int32_t terrain_id, vx_coord coord, i
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (terrain_id < 0 || terrain_id >= (int)ctx->terrains.size()) return VX_ERR_INVALID;
     const int kind = ctx->terrains[terrain_id].kind;
-    if (kind != kDensity && kind != kHeightmap) return VX_ERR_INVALID;
-    const uint64_t n = kind == kDensity ? (uint64_t)edge * edge * edge : (uint64_t)edge * edge;
+    if (kind != kDensity && kind != kHeightmap && kind != kReference) return VX_ERR_INVALID;
+    const uint64_t n = kind == kHeightmap ? (uint64_t)edge * edge : (uint64_t)edge * edge * edge;
     if (cap < n) return VX_ERR_CAPACITY;
     vx_chunk tmp;
     tmp.ctx = ctx;

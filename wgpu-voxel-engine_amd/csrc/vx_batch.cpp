@@ -55,7 +55,7 @@ Layout layout_of(int N) {
 double cost_of(const vx_ctx *ctx, const vx_batch_req &r) {
     const double n3 = (double)r.edge * r.edge * r.edge;
     const TerrainDev &T = ctx->terrains[r.terrain_id];
-    if (T.kind == kDensity) return n3 * T.octaves;
+    if (T.kind == kDensity || T.kind == kReference) return n3 * T.octaves;
     if (T.kind == kHeightmap) return (double)r.edge * r.edge * T.octaves * 8.0 + n3 / 8.0;
     return n3 / 16.0;
 }
@@ -73,9 +73,29 @@ size_t arena_bytes(uint64_t cap, uint32_t formats, size_t *off_v, size_t *off_i)
     return b ? b : 256;
 }
 
+void drop_graph(vx_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+// (Re)allocates the mesh arena for `cap` faces.  The old arena is written by
+// queued runs and its pointers are baked into the captured graph, so the
+// stream is drained and the graph dropped before it is freed.  On failure
+// the batch holds no arena (capacity 0: a later run writes no faces and
+// reports overflow) and no graph.
 int alloc_arena(vx_batch *b, uint64_t cap) {
     vx_ctx *ctx = b->ctx;
-    if (b->arena) ctx->dev_free(b->arena);
+    if (b->arena) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        drop_graph(b);
+        if (e != hipSuccess) {
+            ctx->set_error(e, "arena reallocation");
+            return VX_ERR_DEVICE;
+        }
+        ctx->dev_free(b->arena);
+    }
     b->arena = nullptr;
     size_t off_v, off_i;
     const size_t bytes = arena_bytes(cap, b->formats, &off_v, &off_i);
@@ -98,10 +118,34 @@ int alloc_arena(vx_batch *b, uint64_t cap) {
     return VX_OK;
 }
 
+// VX_CHECK_SCHEDULES=1 (a debugging aid): every vx_batch_wait also reads
+// the schedules back and checks that each is a permutation of the requests.
+bool check_schedules() {
+    static const bool on = [] {
+        const char *v = getenv("VX_CHECK_SCHEDULES");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+
+bool schedules_ok(vx_batch *b) {
+    std::vector<int32_t> h((size_t)b->n);
+    const int32_t *arrays[2] = {b->emit_order, b->order_dev};
+    for (const int32_t *a : arrays) {
+        if (!a) continue;
+        if (hipMemcpy(h.data(), a, (size_t)b->n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (!is_schedule(h.data(), b->n)) return false;
+    }
+    return true;
+}
+
 hipError_t enqueue_fill(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
     return launch_fill(b->fill_dev, b->order_dev, b->n, b->max_edge, ctx->terrains_dev,
-                       ctx->lut_dev, ctx->stream);
+                       ctx->lut_dev, b->overflow + 1, ctx->stream);
 }
 hipError_t enqueue_count(vx_batch *b) {
     return launch_count(b->mesh_dev, b->n, b->max_edge, b->counts, b->ctx->stream);
@@ -144,13 +188,6 @@ hipError_t enqueue_all(vx_batch *b) {
     if (e == hipSuccess) e = enqueue_scan(b);
     if (e == hipSuccess) e = enqueue_emit(b);
     return e;
-}
-
-void drop_graph(vx_batch *b) {
-    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->graph_exec = nullptr;
-    b->graph = nullptr;
 }
 
 int capture_graph(vx_batch *b) {
@@ -324,8 +361,9 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
     b->counts = (uint32_t *)b->ctl;
     b->bases = (uint64_t *)((char *)b->ctl + up256((size_t)n * 4));
     b->total = (uint64_t *)((char *)b->bases + up256((size_t)n * 8));
-    b->overflow = (uint32_t *)(b->total + 1);
+    b->overflow = (uint32_t *)(b->total + 1);  // [0] arena overflow, [1] schedule error
     b->emit_order = (int32_t *)((char *)b->total + 256);
+    VX_HIP(ctx, hipMemsetAsync(b->overflow, 0, 8, ctx->stream));
 
     std::vector<FillWork> fw(n);
     std::vector<MeshWork> mw(n);
@@ -367,9 +405,12 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
         }
     }
     // fused form: every in-batch neighbour must be the adjacent chunk with the
-    // same terrain, so its facing plane can be recomputed from noise
+    // same terrain, so its facing plane can be recomputed from noise; the
+    // reference-rule terrain is generated by the fill kernels only
     static const int ddx[4] = {-1, 1, 0, 0}, ddz[4] = {0, 0, -1, 1};
     b->fused_ok = true;
+    for (int i = 0; i < n; i++)
+        if (ctx->terrains[reqs[i].terrain_id].kind == kReference) b->fused_ok = false;
     for (int i = 0; i < n && b->fused_ok; i++)
         for (int k = 0; k < 4; k++) {
             const int j = reqs[i].nb[k];
@@ -422,6 +463,10 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(),
                          [&](int a, int c) { return cost[a] > cost[c]; });
+        if (!is_schedule(order.data(), n)) {
+            ctx->last_error = "batch fill schedule is not a permutation of the requests";
+            return fail(VX_ERR_DEVICE);
+        }
         if ((st = ctx->dev_alloc((size_t)n * 4, (void **)&b->order_dev))) return fail(st);
         VX_HIP(ctx, hipMemcpyAsync(b->order_dev, order.data(), (size_t)n * 4,
                                    hipMemcpyHostToDevice, ctx->stream));
@@ -502,7 +547,7 @@ extern "C" int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *b) {
     const int n = b->n;
     for (int i = 0; i < n; i++)
         VX_HIP(ctx, launch_fill(b->fill_dev + i, nullptr, 1, b->reqs[i].edge, ctx->terrains_dev,
-                                ctx->lut_dev, ctx->stream));
+                                ctx->lut_dev, b->overflow + 1, ctx->stream));
     for (int i = 0; i < n; i++)
         VX_HIP(ctx, launch_count(b->mesh_dev + i, 1, b->reqs[i].edge, b->counts + i, ctx->stream));
     VX_HIP(ctx, enqueue_scan(b));
@@ -519,8 +564,14 @@ extern "C" int vx_batch_wait(vx_ctx *ctx, vx_batch *b, uint64_t *total_faces) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (!b->ran) return VX_ERR_NOT_READY;
     VX_HIP(ctx, hipMemcpyAsync(ctx->pinned, b->total, 8, hipMemcpyDeviceToHost, ctx->stream));
-    VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 8, hipMemcpyDeviceToHost, ctx->stream));
     VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t sched_err;
+    memcpy(&sched_err, ctx->pinned + 12, 4);
+    if (sched_err || (check_schedules() && !schedules_ok(b))) {
+        ctx->last_error = "batch: a fill or emit launch met a schedule entry outside the batch";
+        return VX_ERR_DEVICE;
+    }
     uint32_t ferr = 0;
     if (use_fused(b)) {
         VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 16, b->fctl_ptrs.error, 4, hipMemcpyDeviceToHost,
@@ -541,15 +592,26 @@ extern "C" int vx_batch_wait(vx_ctx *ctx, vx_batch *b, uint64_t *total_faces) {
         // output: the scan again (same bases from this run's counts; it also
         // writes the emission order, which a fused run never wrote, and clears
         // the overflow word), then the staged emit
-        int st = alloc_arena(b, total + total / 16 + 1024);
-        if (st) return st;
-        drop_graph(b);  // arena pointers are baked into the captured graph
+        int st = alloc_arena(b, total + total / 16 + 1024);  // also drops the graph
+        if (st) {
+            // the run's meshes are lost with the old arena: nothing is
+            // readable until a later run succeeds
+            b->ran = false;
+            b->h_total = 0;
+            b->results = false;
+            return st;
+        }
         VX_HIP(ctx, enqueue_scan(b));
         VX_HIP(ctx, enqueue_emit(b));
-        VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 4, hipMemcpyDeviceToHost,
+        VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 8, hipMemcpyDeviceToHost,
                                    ctx->stream));
         VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
         memcpy(&overflow, ctx->pinned + 8, 4);
+        memcpy(&sched_err, ctx->pinned + 12, 4);
+        if (sched_err) {
+            ctx->last_error = "batch: the re-emit met a schedule entry outside the batch";
+            return VX_ERR_DEVICE;
+        }
         if (overflow) return VX_ERR_CAPACITY;
     }
     b->h_total = total;
@@ -637,7 +699,10 @@ extern "C" int vx_batch_set_cull(vx_batch *b, const vx_frustum *f, const int32_t
     vx_ctx *ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (n == 0) {
-        if (b->cull.n != 0) drop_graph(b);
+        if (b->cull.n != 0) {
+            VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a queued replay may still run
+            drop_graph(b);
+        }
         b->cull = CullJob{};
         return VX_OK;
     }
@@ -655,8 +720,12 @@ extern "C" int vx_batch_set_cull(vx_batch *b, const vx_frustum *f, const int32_t
     VX_HIP(ctx, hipEventRecord(b->planes_ev, ctx->stream));
     CullJob job{b->planes_dev, coords_dev, mask_dev, n, edge};
     if (job.coords != b->cull.coords || job.mask != b->cull.mask || job.n != b->cull.n ||
-        job.edge != b->cull.edge)
-        drop_graph(b);  // kernel arguments change; planes alone are read from memory
+        job.edge != b->cull.edge) {
+        // kernel arguments change (planes alone are read from memory); a
+        // queued replay of the old graph may still run
+        VX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        drop_graph(b);
+    }
     b->cull = job;
     return VX_OK;
 }
@@ -708,6 +777,7 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
         for (int per = 2; per <= 4; per++)
             if (cus * per < b->n) cands.push_back({VX_BATCH_FUSED, cus * per});
     }
+    VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a queued replay may still run
     hipEvent_t e0, e1;
     VX_HIP(ctx, hipEventCreate(&e0));
     VX_HIP(ctx, hipEventCreate(&e1));

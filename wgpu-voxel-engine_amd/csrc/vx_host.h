@@ -11,6 +11,7 @@
 
 #include "../../include/voxel.h"
 #include "vx_internal.h"
+#include "vx_schedule.h"
 
 namespace vx {
 
@@ -24,7 +25,11 @@ class DeviceHeap {
   public:
     static size_t size_class(size_t n);
     hipError_t alloc(size_t n, void **out);
-    void release(void *p);
+    // Released blocks wait here until the caller has seen every stream
+    // that may still use them pass the release (flush_pending).
+    void release_pending(void *p);
+    bool has_pending() const { return !pending_.empty(); }
+    void flush_pending();
     void trim();
     uint64_t live() const { return live_; }
     uint64_t cached() const { return cached_; }
@@ -33,6 +38,7 @@ class DeviceHeap {
   private:
     std::map<size_t, std::vector<void *>> free_;
     std::unordered_map<void *, size_t> sizes_;
+    std::vector<void *> pending_;
     uint64_t live_ = 0, cached_ = 0;
 };
 
@@ -75,6 +81,7 @@ struct vx_ctx {
     void *scratch = nullptr;    // staging for host-array entry points
     size_t scratch_bytes = 0;
     unsigned char *pinned = nullptr;  // 4 KiB pinned host staging for readbacks
+    hipEvent_t free_fence = nullptr;  // recorded on `stream` at every dev_free
     uint32_t *draw_counts = nullptr;  // per-workgroup counts of draw-list passes
     int32_t draw_counts_cap = 0;
 

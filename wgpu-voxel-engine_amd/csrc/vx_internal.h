@@ -15,7 +15,7 @@ constexpr int kEmitGrid = 1280;    // emit workgroups that fill the GPU (5 per C
 constexpr int kFillGrid = 1024;    // fill workgroups that fill the GPU (4 per CU)
 
 // Terrain kinds (mirror of vx_terrain.kind in include/voxel.h).
-enum : int32_t { kAir = 0, kSolid = 1, kHeightmap = 2, kDensity = 3 };
+enum : int32_t { kAir = 0, kSolid = 1, kHeightmap = 2, kDensity = 3, kReference = 4 };
 
 // Block ids stored in the u8 voxel array: 0 air, 1 grass, 2 dirt.  The
 // reference's CubeType is {GRASS=0, DIRT=1, AIR=2} (src/cube.rs:16-21).
@@ -39,13 +39,15 @@ struct TerrainDev {
 };
 
 // Per-hash gradient coefficients for the column-hoisted noise evaluation,
-// indexed [corner k][hash & 15].  grad_k = fma(yr, gy, ax*xr + az*zr); the
-// +-1 entries select and sign an operand, the zero entries are the
-// signed zero that makes the product vanish exactly (-0 when the operand is
-// >= 0, +0 when it is < 0), so the sum equals Perlin's (+-u) + (+-v) bit for
-// bit.
+// indexed [corner k][hash & 15] with corner k = x | row << 1 | col << 2 (row
+// = world y, the per-row axis; col = world z).  grad_k = fma(yr, g_row,
+// a_x*xr + a_col*zr); the +-1 entries select and sign an operand, the zero
+// entries are the signed zero that makes the product vanish exactly (-0
+// when the operand is >= 0, +0 when it is < 0), so the sum equals Perlin's
+// (+-u) + (+-v) bit for bit.
 struct GradLut {
-    float4 c[8][16];  // (ax, az, gy, unused)
+    float4 c[8][16];  // noise at (wx, wy, wz): (a_x, a_col, g_row) = (gx, gz, gy)
+    float4 r[8][16];  // noise at (wx, wz, wy), the reference's axes: (gx, gy, gz)
 };
 
 // Meshing constants computed on the host from the rotation of each face
@@ -147,8 +149,13 @@ enum : uint32_t { kDrawInstanced = 1u, kDrawExpanded = 2u };
 inline int words_per_row(int edge) { return (edge + 31) / 32; }
 
 // ---- kernel launchers (vx_kernels.hip) --------------------------------
+// Schedules: `order` (fill, emit) lists the n items in launch order, or is
+// null for 0..n-1.  An entry outside [0, n) makes its workgroup do nothing
+// and sets a sticky error word: `error` for the fill, overflow[1] for the
+// emit, whose overflow[0] is the arena-overflow flag the scan clears.
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
-                       const TerrainDev *terrains, const GradLut *lut, hipStream_t s);
+                       const TerrainDev *terrains, const GradLut *lut, uint32_t *error,
+                       hipStream_t s);
 hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint32_t *counts,
                         hipStream_t s);
 hipError_t launch_scan(const uint32_t *counts, int32_t n, uint64_t *bases, uint64_t *total,

@@ -165,14 +165,64 @@ __device__ __forceinline__ void store_lines(SH &S, const Halo &H, uint64_t b, in
     }
 }
 
+// The 8 corner hashes of a lattice cell for the column-hoisted evaluation,
+// indexed by corner k = x | row << 1 | col << 2 (row: world y, the per-row
+// axis; col: world z, the column's second axis).  px0/px1 = perm[X], perm[X+1].
+// Density rule, noise at (wx, wy, wz): Perlin's hash p[p[p[X]+Y]+Z] takes
+// the row cell at its second level and the column cell at its third.
+// Reference rule, noise at (wx, wz, wy) (src/perlin_noise.rs:6-13): the
+// column cell enters at the second level and the row cell at the third.
+template <bool REF, class SH>
+__device__ __forceinline__ void corner_hashes(const SH &S, int px0, int px1, int yrow, int zcol,
+                                              int h[8]) {
+    if (!REF) {
+        const int A = px0 + yrow, B = px1 + yrow;
+        const int AA = S.perm[A] + zcol, AB = S.perm[A + 1] + zcol;
+        const int BA = S.perm[B] + zcol, BB = S.perm[B + 1] + zcol;
+        h[0] = S.perm[AA]; h[1] = S.perm[BA]; h[2] = S.perm[AB]; h[3] = S.perm[BB];
+        h[4] = S.perm[AA + 1]; h[5] = S.perm[BA + 1]; h[6] = S.perm[AB + 1]; h[7] = S.perm[BB + 1];
+    } else {
+        const int A = px0 + zcol, B = px1 + zcol;
+        const int AA = S.perm[A] + yrow, AB = S.perm[A + 1] + yrow;
+        const int BA = S.perm[B] + yrow, BB = S.perm[B + 1] + yrow;
+        h[0] = S.perm[AA]; h[1] = S.perm[BA]; h[2] = S.perm[AA + 1]; h[3] = S.perm[BA + 1];
+        h[4] = S.perm[AB]; h[5] = S.perm[BB]; h[6] = S.perm[AB + 1]; h[7] = S.perm[BB + 1];
+    }
+}
+
+// Trilinear blend of the 8 corner gradients (indexed as in corner_hashes)
+// in Perlin's order: x innermost, then the noise's y axis, then its z axis.
+// u: x fade; wz: column (world z) fade; wy: row (world y) fade.
+template <bool REF>
+__device__ __forceinline__ float blend8(float u, float wz, float wy, float g0, float g1, float g2,
+                                        float g3, float g4, float g5, float g6, float g7) {
+    if (!REF)  // noise y = world y (row), noise z = world z (column)
+        return lerp(wz, lerp(wy, lerp(u, g0, g1), lerp(u, g2, g3)),
+                    lerp(wy, lerp(u, g4, g5), lerp(u, g6, g7)));
+    // noise y = world z (column), noise z = world y (row)
+    return lerp(wy, lerp(wz, lerp(u, g0, g1), lerp(u, g4, g5)),
+                lerp(wz, lerp(u, g2, g3), lerp(u, g6, g7)));
+}
+
+// The fill rule on a voxel's fBm sum: density terrains are solid iff
+// fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0; the reference rule
+// (src/chunk.rs:38-42) is solid iff fbm3(wx*f, wz*f, wy*f) < 0.
+template <bool REF>
+__device__ __forceinline__ bool solid_of(float sum, const TerrainDev &T, int wy) {
+    if (REF) return sum < 0.0f;
+    const float thr = (float)wy * T.y_scale;
+    return (sum - thr) > 0.0f;
+}
+
 // 3-D fBm density for every line: the column's x/z lattice terms are hoisted
 // per octave, the y terms come from S.ytab, the 8 corner gradients are
 // rehashed only when the y lattice cell changes.  Voxel is solid iff
-// fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0 (oracle: vxo_chunk_fill).
+// fbm3(wx*f, wy*f, wz*f) - wy*y_scale > 0 (REF: fbm3(wx*f, wz*f, wy*f) < 0;
+// oracle: vxo_chunk_fill).
 // Work items are (line group, y segment of SEG rows) pairs dealt round-robin
 // to the waves, so that the halo lines do not leave some waves a whole extra
 // line group (N = 32 with four halo lines: 36 items of 16 rows, 9 per wave).
-template <int N, int MAXN, bool DBG, int SEG, class SH>
+template <int N, int MAXN, bool DBG, int SEG, bool REF, class SH>
 __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
                               float *noise_out) {
     constexpr int L = 64 / N;
@@ -214,12 +264,8 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
                     const int Yi = __builtin_amdgcn_readfirstlane(__float_as_int(t.w));
                     if (Yi != yprev) {  // new lattice cell along y: rehash 8 corners
                         yprev = Yi;
-                        const int A = pX0 + Yi, B = pX1 + Yi;
-                        const int AA = S.perm[A] + Zi, AB = S.perm[A + 1] + Zi;
-                        const int BA = S.perm[B] + Zi, BB = S.perm[B + 1] + Zi;
-                        const int h[8] = {S.perm[AA],     S.perm[BA],     S.perm[AB],
-                                          S.perm[BB],     S.perm[AA + 1], S.perm[BA + 1],
-                                          S.perm[AB + 1], S.perm[BB + 1]};
+                        int h[8];
+                        corner_hashes<REF>(S, pX0, pX1, Yi, Zi, h);
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
                             const float4 Lk = S.lut[k][h[k] & 15];
@@ -233,8 +279,7 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
                     const float g2 = fmaf(t.y, gy[2], c[2]), g3 = fmaf(t.y, gy[3], c[3]);
                     const float g4 = fmaf(t.x, gy[4], c[4]), g5 = fmaf(t.x, gy[5], c[5]);
                     const float g6 = fmaf(t.y, gy[6], c[6]), g7 = fmaf(t.y, gy[7], c[7]);
-                    const float n = lerp(wz, lerp(t.z, lerp(u, g0, g1), lerp(u, g2, g3)),
-                                         lerp(t.z, lerp(u, g4, g5), lerp(u, g6, g7)));
+                    const float n = blend8<REF>(u, wz, t.z, g0, g1, g2, g3, g4, g5, g6, g7);
                     sum[yy] = sum[yy] + amp * n;
                 }
                 f = f * T.lacunarity;
@@ -243,8 +288,7 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
 #pragma unroll 1
             for (int yy = 0; yy < SEG; yy++) {
                 const int y = y0 + yy;
-                const float thr = (float)(oy + y) * T.y_scale;
-                const bool solid = (sum[yy] - thr) > 0.0f;
+                const bool solid = solid_of<REF>(sum[yy], T, oy + y);
                 store_lines<N, MAXN>(S, H, __ballot(solid), y, g, lane);
                 if (DBG && li < N)
                     noise_out[(size_t)(wxi - ox) + (size_t)N * li + (size_t)N * N * y] = sum[yy];
@@ -261,7 +305,7 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
 // terms (fade weights, x permutation entries, z cell) and the 8 corner
 // coefficients stay in registers; the lattice fractions are recomputed in
 // the rare rehash.  Octaves are summed in the same order as the oracle.
-template <int N, int MAXN, bool DBG, int SEG, int OCT, class SH>
+template <int N, int MAXN, bool DBG, int SEG, int OCT, bool REF, class SH>
 __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
                                   SH &S, float *noise_out, int seg_only = -1) {
     constexpr int L = 64 / N;
@@ -328,12 +372,8 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
 #pragma unroll
                 for (int q = 1; q < OCT; q++)
                     if (o == q) px0 = pX0[q], px1 = pX1[q], zi = Zi[q];
-                const int A = px0 + Yi, B = px1 + Yi;
-                const int AA = S.perm[A] + zi, AB = S.perm[A + 1] + zi;
-                const int BA = S.perm[B] + zi, BB = S.perm[B + 1] + zi;
-                const int h[8] = {S.perm[AA],     S.perm[BA],     S.perm[AB],
-                                  S.perm[BB],     S.perm[AA + 1], S.perm[BA + 1],
-                                  S.perm[AB + 1], S.perm[BB + 1]};
+                int h[8];
+                corner_hashes<REF>(S, px0, px1, Yi, zi, h);
                 float cn[8], gn[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
@@ -368,12 +408,10 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
                 const float g2 = fmaf(t.y, gy[o][2], c[o][2]), g3 = fmaf(t.y, gy[o][3], c[o][3]);
                 const float g4 = fmaf(t.x, gy[o][4], c[o][4]), g5 = fmaf(t.x, gy[o][5], c[o][5]);
                 const float g6 = fmaf(t.y, gy[o][6], c[o][6]), g7 = fmaf(t.y, gy[o][7], c[o][7]);
-                const float n = lerp(wz[o], lerp(t.z, lerp(u[o], g0, g1), lerp(u[o], g2, g3)),
-                                     lerp(t.z, lerp(u[o], g4, g5), lerp(u[o], g6, g7)));
+                const float n = blend8<REF>(u[o], wz[o], t.z, g0, g1, g2, g3, g4, g5, g6, g7);
                 sum = sum + ao[o] * n;
             }
-            const float thr = (float)(oy + y) * T.y_scale;
-            store_lines<N, MAXN>(S, H, __ballot((sum - thr) > 0.0f), y, g, lane);
+            store_lines<N, MAXN>(S, H, __ballot(solid_of<REF>(sum, T, oy + y)), y, g, lane);
             if (DBG && li < N)
                 noise_out[(size_t)(wxi - ox) + (size_t)N * li + (size_t)N * N * y] = sum;
         }
@@ -440,6 +478,22 @@ __device__ __forceinline__ void build_ytab(const TerrainDev &T, int oy, SH &S) {
 // OCT_INNER: the octave-inner density loop for 1..4 octaves (the fill
 // kernels); the fused kernel keeps the octave-outer loop, which needs fewer
 // registers next to its emit phase.
+template <int N, int MAXN, bool DBG, int SEG, bool REF, class SH>
+__device__ __forceinline__ void density_rows(const TerrainDev &T, int ox, int oy, int oz,
+                                             const Halo &H, SH &S, float *noise_out) {
+    build_ytab<N>(T, oy, S);
+    __syncthreads();
+    switch (T.octaves) {
+    case 1: density_lines_oct<N, MAXN, DBG, SEG, 1, REF>(T, ox, oy, oz, H, S, noise_out); break;
+    case 2: density_lines_oct<N, MAXN, DBG, SEG, 2, REF>(T, ox, oy, oz, H, S, noise_out); break;
+    case 3: density_lines_oct<N, MAXN, DBG, SEG, 3, REF>(T, ox, oy, oz, H, S, noise_out); break;
+    case 4: density_lines_oct<N, MAXN, DBG, SEG, 4, REF>(T, ox, oy, oz, H, S, noise_out); break;
+    default: density_lines<N, MAXN, DBG, SEG, REF>(T, ox, oy, oz, H, S, noise_out); break;
+    }
+}
+
+// The reference rule (kReference) is generated by the fill kernels only; a
+// batch holding such a terrain never takes the fused form.
 template <int N, int MAXN, bool DBG, int SEG, bool OCT_INNER, class SH>
 __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
                               float *noise_out) {
@@ -447,17 +501,11 @@ __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const
     if (T.kind == kDensity && !OCT_INNER) {
         build_ytab<N>(T, oy, S);
         __syncthreads();
-        density_lines<N, MAXN, DBG, SEG>(T, ox, oy, oz, H, S, noise_out);
+        density_lines<N, MAXN, DBG, SEG, false>(T, ox, oy, oz, H, S, noise_out);
     } else if (T.kind == kDensity) {
-        build_ytab<N>(T, oy, S);
-        __syncthreads();
-        switch (T.octaves) {
-        case 1: density_lines_oct<N, MAXN, DBG, SEG, 1>(T, ox, oy, oz, H, S, noise_out); break;
-        case 2: density_lines_oct<N, MAXN, DBG, SEG, 2>(T, ox, oy, oz, H, S, noise_out); break;
-        case 3: density_lines_oct<N, MAXN, DBG, SEG, 3>(T, ox, oy, oz, H, S, noise_out); break;
-        case 4: density_lines_oct<N, MAXN, DBG, SEG, 4>(T, ox, oy, oz, H, S, noise_out); break;
-        default: density_lines<N, MAXN, DBG, SEG>(T, ox, oy, oz, H, S, noise_out); break;
-        }
+        density_rows<N, MAXN, DBG, SEG, false>(T, ox, oy, oz, H, S, noise_out);
+    } else if (OCT_INNER && T.kind == kReference) {
+        density_rows<N, MAXN, DBG, SEG, true>(T, ox, oy, oz, H, S, noise_out);
     } else if (T.kind == kHeightmap) {
         heightmap_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
     } else {
@@ -526,13 +574,13 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
 
 // Rows [part * N / parts, +N / parts) of chunk `w` through the octave-inner
 // density loop, then their outputs (one of `parts` workgroups on the chunk).
-template <int N, int MAXN, int PARTS, int OCT>
+template <int N, int MAXN, int PARTS, int OCT, bool REF>
 __device__ void fill_part(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S, int part) {
     const Halo H{0, 0u};
     build_ytab<N>(T, w.cy * N, S);
     __syncthreads();
-    density_lines_oct<N, MAXN, false, N / PARTS, OCT>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
-                                                      nullptr, part);
+    density_lines_oct<N, MAXN, false, N / PARTS, OCT, REF>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
+                                                           nullptr, part);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass, part * (N / PARTS),
                                  (part + 1) * (N / PARTS));
@@ -547,40 +595,47 @@ __device__ __forceinline__ void fill_chunk(const FillWork &w, const TerrainDev &
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
 }
 
-// One of `parts` (2, 4 or 8) workgroups on a chunk: density terrains with
-// up to 4 octaves split their rows among the parts (8 only at edge 64; a
-// smaller chunk uses the first 4); any other terrain is filled whole by part
-// 0 while the other parts return.
+template <int N, int MAXN, bool REF>
+__device__ __forceinline__ void fill_parts(const FillWork &w, const TerrainDev &T,
+                                           FillShared<MAXN> &S, int part, int parts) {
+    if (parts == 8) {
+        switch (T.octaves) {
+        case 1: fill_part<N, MAXN, (N == 64 ? 8 : 4), 1, REF>(w, T, S, part); break;
+        case 2: fill_part<N, MAXN, (N == 64 ? 8 : 4), 2, REF>(w, T, S, part); break;
+        case 3: fill_part<N, MAXN, (N == 64 ? 8 : 4), 3, REF>(w, T, S, part); break;
+        default: fill_part<N, MAXN, (N == 64 ? 8 : 4), 4, REF>(w, T, S, part); break;
+        }
+    } else if (parts == 2) {
+        switch (T.octaves) {
+        case 1: fill_part<N, MAXN, 2, 1, REF>(w, T, S, part); break;
+        case 2: fill_part<N, MAXN, 2, 2, REF>(w, T, S, part); break;
+        case 3: fill_part<N, MAXN, 2, 3, REF>(w, T, S, part); break;
+        default: fill_part<N, MAXN, 2, 4, REF>(w, T, S, part); break;
+        }
+    } else {
+        switch (T.octaves) {
+        case 1: fill_part<N, MAXN, 4, 1, REF>(w, T, S, part); break;
+        case 2: fill_part<N, MAXN, 4, 2, REF>(w, T, S, part); break;
+        case 3: fill_part<N, MAXN, 4, 3, REF>(w, T, S, part); break;
+        default: fill_part<N, MAXN, 4, 4, REF>(w, T, S, part); break;
+        }
+    }
+}
+
+// One of `parts` (2, 4 or 8) workgroups on a chunk: density terrains (either
+// rule) with up to 4 octaves split their rows among the parts (8 only at
+// edge 64; a smaller chunk uses the first 4); any other terrain is filled
+// whole by part 0 while the other parts return.
 template <int N, int MAXN>
 __device__ __forceinline__ void fill_chunk_split(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S,
                                  int part, int parts) {
-    if (T.kind == kDensity && T.octaves <= 4) {
+    if ((T.kind == kDensity || T.kind == kReference) && T.octaves <= 4) {
         if (parts == 8 && N < 64) {
             if (part >= 4) return;
             parts = 4;
         }
-        if (parts == 8) {
-            switch (T.octaves) {
-            case 1: fill_part<N, MAXN, (N == 64 ? 8 : 4), 1>(w, T, S, part); break;
-            case 2: fill_part<N, MAXN, (N == 64 ? 8 : 4), 2>(w, T, S, part); break;
-            case 3: fill_part<N, MAXN, (N == 64 ? 8 : 4), 3>(w, T, S, part); break;
-            default: fill_part<N, MAXN, (N == 64 ? 8 : 4), 4>(w, T, S, part); break;
-            }
-        } else if (parts == 2) {
-            switch (T.octaves) {
-            case 1: fill_part<N, MAXN, 2, 1>(w, T, S, part); break;
-            case 2: fill_part<N, MAXN, 2, 2>(w, T, S, part); break;
-            case 3: fill_part<N, MAXN, 2, 3>(w, T, S, part); break;
-            default: fill_part<N, MAXN, 2, 4>(w, T, S, part); break;
-            }
-        } else {
-            switch (T.octaves) {
-            case 1: fill_part<N, MAXN, 4, 1>(w, T, S, part); break;
-            case 2: fill_part<N, MAXN, 4, 2>(w, T, S, part); break;
-            case 3: fill_part<N, MAXN, 4, 3>(w, T, S, part); break;
-            default: fill_part<N, MAXN, 4, 4>(w, T, S, part); break;
-            }
-        }
+        if (T.kind == kReference) fill_parts<N, MAXN, true>(w, T, S, part, parts);
+        else fill_parts<N, MAXN, false>(w, T, S, part, parts);
         return;
     }
     if (part == 0) fill_chunk<N, MAXN, false>(w, T, S);
@@ -625,8 +680,9 @@ __device__ __forceinline__ void fill_item(const FillWork &w, const TerrainDev *t
                                           const GradLut *lut, FillShared<MAXN> &S, int part = 0,
                                           int parts = 1) {
     const TerrainDev &T = terrains[w.terrain];
+    const float4(*L)[16] = T.kind == kReference ? lut->r : lut->c;
     for (int i = threadIdx.x; i < 512; i += kBlock) S.perm[i] = T.perm[i];
-    for (int i = threadIdx.x; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
+    for (int i = threadIdx.x; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = L[i >> 4][i & 15];
     __syncthreads();
     if (w.noise_out) {  // parity-check path: also stores the raw noise
         if (SPLIT && part > 0) return;
@@ -658,12 +714,22 @@ __device__ __forceinline__ void fill_item(const FillWork &w, const TerrainDev *t
 // Four waves per SIMD (at most 128 VGPRs) for edges up to 32: left to
 // itself the allocator takes one register more and drops to three waves,
 // about 15% slower.  Edge-64 batches stay at three waves.
+// A schedule entry outside [0, n) (a corrupted order array) sets the sticky
+// error word and the workgroup does nothing: no access is made through it.
+__device__ __forceinline__ bool order_entry_ok(int item, int n, uint32_t *error) {
+    if ((unsigned)item < (unsigned)n) return true;
+    if (threadIdx.x == 0) atomicOr(error, 1u);
+    return false;
+}
+
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
-fill_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order,
-            const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut) {
+fill_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order, int32_t n,
+            const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut,
+            uint32_t *__restrict__ error) {
     __shared__ FillShared<MAXN> S;
     const int item = order ? order[blockIdx.x] : (int)blockIdx.x;
+    if (!order_entry_ok(item, n, error)) return;
     fill_item<MAXN>(work[item], terrains, lut, S);
 }
 
@@ -671,12 +737,13 @@ fill_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order
 // the row-split path does not add to the registers of fill_kernel).
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
-fill_split_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order,
+fill_split_kernel(const FillWork *__restrict__ work, const int32_t *__restrict__ order, int32_t n,
                   const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut,
-                  int parts) {
+                  int parts, uint32_t *__restrict__ error) {
     __shared__ FillShared<MAXN> S;
     const int k = (int)blockIdx.x / parts, part = (int)blockIdx.x - k * parts;
     const int item = order ? order[k] : k;
+    if (!order_entry_ok(item, n, error)) return;
     fill_item<MAXN, true>(work[item], terrains, lut, S, part, parts);
 }
 
@@ -882,7 +949,7 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     }
     if (tid == 0) {
         *total = sum;
-        if (overflow) *overflow = 0u;  // the emit stage that follows sets it
+        if (overflow) *overflow = 0u;  // the emit stage that follows sets it (not overflow[1])
     }
     if (!order_out) return;
     static_assert(kOrderBuckets == 1024, "one bucket per thread");
@@ -1148,10 +1215,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
                                                       MeshOut out,
                                                       const MeshTables *__restrict__ tables,
                                                       uint32_t *__restrict__ overflow,
-                                                      int parts) {
+                                                      int parts, int32_t n) {
     __shared__ MeshShared<MAXN> S;
     const int k = (int)blockIdx.x / parts, part = (int)blockIdx.x - k * parts;
     const int item = order ? order[k] : k;
+    if (!order_entry_ok(item, n, overflow + 1)) return;
     emit_item<MAXN>(work[item], bases[item], counts[item], out, tables, overflow, S, part, parts);
 }
 
@@ -1466,7 +1534,8 @@ __global__ void __launch_bounds__(kBlock) draw_write_kernel(Planes planes,
 
 // --------------------------------------------------------------- launchers
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
-                       const TerrainDev *terrains, const GradLut *lut, hipStream_t s) {
+                       const TerrainDev *terrains, const GradLut *lut, uint32_t *error,
+                       hipStream_t s) {
     if (n <= 0) return hipSuccess;
     // small batches: 2, 4 or 8 workgroups per chunk, so the grid still
     // covers about kFillGrid workgroups (density terrains with up to 4
@@ -1477,15 +1546,18 @@ hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, in
                       : 2 * n >= kFillGrid ? 2
                       : (max_edge > 32 && 8 * n <= kFillGrid) ? 8 : 4;
     if (parts == 1) {
-        if (max_edge <= 16) fill_kernel<16><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
-        else if (max_edge <= 32) fill_kernel<32><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
-        else fill_kernel<64><<<n, kBlock, 0, s>>>(work, order, terrains, lut);
+        if (max_edge <= 16) fill_kernel<16><<<n, kBlock, 0, s>>>(work, order, n, terrains, lut, error);
+        else if (max_edge <= 32) fill_kernel<32><<<n, kBlock, 0, s>>>(work, order, n, terrains, lut, error);
+        else fill_kernel<64><<<n, kBlock, 0, s>>>(work, order, n, terrains, lut, error);
         return hipGetLastError();
     }
     const int grid = n * parts;
-    if (max_edge <= 16) fill_split_kernel<16><<<grid, kBlock, 0, s>>>(work, order, terrains, lut, parts);
-    else if (max_edge <= 32) fill_split_kernel<32><<<grid, kBlock, 0, s>>>(work, order, terrains, lut, parts);
-    else fill_split_kernel<64><<<grid, kBlock, 0, s>>>(work, order, terrains, lut, parts);
+    if (max_edge <= 16)
+        fill_split_kernel<16><<<grid, kBlock, 0, s>>>(work, order, n, terrains, lut, parts, error);
+    else if (max_edge <= 32)
+        fill_split_kernel<32><<<grid, kBlock, 0, s>>>(work, order, n, terrains, lut, parts, error);
+    else
+        fill_split_kernel<64><<<grid, kBlock, 0, s>>>(work, order, n, terrains, lut, parts, error);
     return hipGetLastError();
 }
 
@@ -1547,11 +1619,11 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
         n >= kEmitGrid ? 1 : std::min(max_edge > 32 ? 8 : 4, (kEmitGrid + n - 1) / n);
     const int grid = n * parts;
     if (max_edge <= 16)
-        emit_kernel<16><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
+        emit_kernel<16><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts, n);
     else if (max_edge <= 32)
-        emit_kernel<32><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
+        emit_kernel<32><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts, n);
     else
-        emit_kernel<64><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts);
+        emit_kernel<64><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts, n);
     return hipGetLastError();
 }
 

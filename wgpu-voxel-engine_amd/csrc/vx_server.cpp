@@ -48,10 +48,12 @@ uint64_t now_ns() {
 
 size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// A resident or in-flight chunk is identified by its coordinate, edge and
+// terrain: the same coordinate under two terrains is two different chunks.
 struct Key {
-    int32_t x, y, z, edge;
+    int32_t x, y, z, edge, terrain;
     bool operator==(const Key &o) const {
-        return x == o.x && y == o.y && z == o.z && edge == o.edge;
+        return x == o.x && y == o.y && z == o.z && edge == o.edge && terrain == o.terrain;
     }
 };
 
@@ -61,6 +63,7 @@ struct KeyHash {
         h ^= (uint64_t)(uint32_t)k.y * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
         h ^= (uint64_t)(uint32_t)k.z * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
         h ^= (uint64_t)(uint32_t)k.edge + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)k.terrain * 0xD6E8FEB86659FD93ull + (h << 6) + (h >> 2);
         return (size_t)h;
     }
 };
@@ -241,7 +244,7 @@ int admit(vx_server *S) {
     while (!S->pending.empty() && F->n < S->max_batch) {
         auto [ticket, r] = S->pending.front();
         S->pending.pop_front();
-        const Key k{r.coord.x, r.coord.y, r.coord.z, r.edge};
+        const Key k{r.coord.x, r.coord.y, r.coord.z, r.edge, r.terrain_id};
         auto hit = S->cache.find(k);
         if (hit != S->cache.end()) {
             S->lru.splice(S->lru.begin(), S->lru, hit->second.lru);
@@ -342,7 +345,7 @@ int admit(vx_server *S) {
         for (int k = 0; k < 4; k++) {
             F->nb_present[i][k] = 0;
             if (r.neighbours != VX_NB_RESIDENT) continue;
-            const Key nk{r.coord.x + dx[k], r.coord.y, r.coord.z + dz[k], r.edge};
+            const Key nk{r.coord.x + dx[k], r.coord.y, r.coord.z + dz[k], r.edge, r.terrain_id};
             auto it = S->cache.find(nk);
             if (it == S->cache.end()) continue;
             vx_chunk *nc = it->second.chunk;
@@ -360,14 +363,23 @@ int admit(vx_server *S) {
                                   hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = hipMemcpyAsync(F->mesh_dev, mw, (size_t)n * sizeof(MeshWork), hipMemcpyHostToDevice, s);
+    // overflow flag and sticky schedule-error word of this flight
+    if (e == hipSuccess) e = hipMemsetAsync(F->overflow, 0, 8, s);
+    const char *what = "server admit";
     if (!uniform && e == hipSuccess) {
         for (int i = 0; i < n; i++) ord[i] = i;
         std::stable_sort(ord, ord + n, [&](int a, int b) { return F->reqs[a].edge > F->reqs[b].edge; });
         F->order_dev = (int32_t *)(base + off_order);
-        e = hipMemcpyAsync(F->order_dev, ord, (size_t)n * 4, hipMemcpyHostToDevice, s);
+        if (!is_schedule(ord, n)) {
+            e = hipErrorInvalidValue;
+            what = "server admit: fill schedule is not a permutation of the flight";
+        } else {
+            e = hipMemcpyAsync(F->order_dev, ord, (size_t)n * 4, hipMemcpyHostToDevice, s);
+        }
     }
     if (e == hipSuccess)
-        e = launch_fill(F->fill_dev, F->order_dev, n, F->max_edge, ctx->terrains_dev, ctx->lut_dev, s);
+        e = launch_fill(F->fill_dev, F->order_dev, n, F->max_edge, ctx->terrains_dev, ctx->lut_dev,
+                        F->overflow + 1, s);
     if (e == hipSuccess) e = launch_count(F->mesh_dev, n, F->max_edge, F->counts, s);
     // the scan rewrites the order array (the fill has consumed it) as the
     // emission order, largest chunks first
@@ -375,10 +387,11 @@ int admit(vx_server *S) {
     if (e == hipSuccess)
         e = launch_scan(F->counts, n, F->bases, F->total, F->overflow, F->order_dev, s);
     unsigned char *back = sl.pinned + sl.pinned_bytes - 256;
-    if (e == hipSuccess) e = hipMemcpyAsync(back, F->total, 8, hipMemcpyDeviceToHost, s);
+    // total (8 B) | overflow | schedule error
+    if (e == hipSuccess) e = hipMemcpyAsync(back, F->total, 16, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipEventRecord(sl.ev1, s);
     if (e != hipSuccess) {
-        ctx->set_error(e, "server admit");
+        ctx->set_error(e, what);
         (void)hipStreamSynchronize(s);
         fail_flight_requests(S, F, VX_ERR_DEVICE);
         F->slot = -1;
@@ -407,7 +420,15 @@ int advance(vx_server *S, Flight *F) {
             return VX_ERR_DEVICE;
         }
         uint64_t total;
+        uint32_t sched_err;
         memcpy(&total, sl.pinned + sl.pinned_bytes - 256, 8);
+        memcpy(&sched_err, sl.pinned + sl.pinned_bytes - 256 + 12, 4);
+        if (sched_err) {
+            ctx->last_error = "server: a fill launch met a schedule entry outside the flight";
+            fail_flight_requests(S, F, VX_ERR_DEVICE);
+            release_flight(S, F);
+            return VX_ERR_DEVICE;
+        }
         F->faces = total;
         size_t bytes = 0, off_v = 0, off_i = 0;
         if (S->formats & kFmtInstances) bytes = up256(total * 68);
@@ -442,6 +463,9 @@ int advance(vx_server *S, Flight *F) {
         unsigned char *back = sl.pinned + sl.pinned_bytes - 256 - up256((size_t)S->max_batch * 12);
         if (e == hipSuccess)
             e = hipMemcpyAsync(back, F->counts, (size_t)F->n * 4, hipMemcpyDeviceToHost, sl.stream);
+        if (e == hipSuccess)  // overflow | schedule error after the emit
+            e = hipMemcpyAsync(sl.pinned + sl.pinned_bytes - 256 + 16, F->overflow, 8,
+                               hipMemcpyDeviceToHost, sl.stream);
         if (e == hipSuccess) e = hipEventRecord(sl.ev2, sl.stream);
         if (e != hipSuccess) {
             ctx->set_error(e, "server emit");
@@ -461,6 +485,17 @@ int advance(vx_server *S, Flight *F) {
         fail_flight_requests(S, F, VX_ERR_DEVICE);
         release_flight(S, F);
         return VX_ERR_DEVICE;
+    }
+    {
+        uint32_t flags[2];
+        memcpy(flags, sl.pinned + sl.pinned_bytes - 256 + 16, 8);
+        if (flags[1] || flags[0]) {  // the arena is sized exactly: no overflow either
+            ctx->last_error = flags[1] ? "server: an emit launch met a schedule entry outside the flight"
+                                       : "server: mesh arena overflow";
+            fail_flight_requests(S, F, VX_ERR_DEVICE);
+            release_flight(S, F);
+            return VX_ERR_DEVICE;
+        }
     }
     const uint32_t *counts =
         (const uint32_t *)(sl.pinned + sl.pinned_bytes - 256 - up256((size_t)S->max_batch * 12));
@@ -605,7 +640,7 @@ extern "C" int vx_server_submit(vx_server *S, const vx_request *reqs, int32_t n,
         const uint64_t ticket = S->next_ticket++;
         if (tickets) tickets[i] = ticket;
         S->st.submitted++;
-        const Key k{r.coord.x, r.coord.y, r.coord.z, r.edge};
+        const Key k{r.coord.x, r.coord.y, r.coord.z, r.edge, r.terrain_id};
         auto hit = S->cache.find(k);
         if (hit != S->cache.end()) {  // resident: served without device work
             S->lru.splice(S->lru.begin(), S->lru, hit->second.lru);
@@ -664,11 +699,11 @@ extern "C" int vx_server_drain(vx_server *S) {
     }
 }
 
-extern "C" int vx_server_lookup(vx_server *S, vx_coord c, int32_t edge, vx_chunk **chunk,
-                                vx_mesh **mesh) {
+extern "C" int vx_server_lookup(vx_server *S, vx_coord c, int32_t edge, int32_t terrain_id,
+                                vx_chunk **chunk, vx_mesh **mesh) {
     if (!S) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(S->ctx->mu);
-    auto it = S->cache.find(Key{c.x, c.y, c.z, edge});
+    auto it = S->cache.find(Key{c.x, c.y, c.z, edge, terrain_id});
     if (it == S->cache.end()) return VX_ERR_NOT_READY;
     S->lru.splice(S->lru.begin(), S->lru, it->second.lru);
     if (chunk) {

@@ -28,7 +28,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "voxel.h")
 
 OK, ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_CAPACITY, ERR_BUDGET, ERR_NOT_READY, ERR_NODEV = (
     0, -1, -2, -3, -4, -5, -6, -7)
-TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY = range(4)
+TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY, TERRAIN_REFERENCE = range(5)
 FMT_INSTANCES, FMT_VERTICES = 1, 2
 FMT_INDICES = 4  # vx_batch_read_mesh selector for the index buffer
 BATCH_AUTO, BATCH_FUSED, BATCH_STAGED = 0, 1, 2
@@ -195,7 +195,7 @@ _SIGS = {
     "vx_server_pump": (C.c_int, [P, C.c_int32]),
     "vx_server_poll": (C.c_int, [P, P, C.c_int32, C.POINTER(C.c_int32)]),
     "vx_server_drain": (C.c_int, [P]),
-    "vx_server_lookup": (C.c_int, [P, Coord, C.c_int32, C.POINTER(P), C.POINTER(P)]),
+    "vx_server_lookup": (C.c_int, [P, Coord, C.c_int32, C.c_int32, C.POINTER(P), C.POINTER(P)]),
     "vx_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
     "vx_serve_simulate": (C.c_int, [P, C.POINTER(SimDesc), C.POINTER(SimResult)]),
     "vx_world_create": (C.c_int, [P, C.POINTER(WorldDesc), C.POINTER(P)]),
@@ -302,7 +302,7 @@ class Context:
         return self._terrains[t]
 
     def noise(self, t: Terrain, coord, edge: int) -> np.ndarray:
-        n = edge ** 3 if t.kind == TERRAIN_DENSITY else edge ** 2
+        n = edge ** 2 if t.kind == TERRAIN_HEIGHTMAP else edge ** 3
         out = np.zeros(n, np.float32)
         self.check(self.L.vx_chunk_noise(self.h, self.terrain(t), Coord(*coord), edge, _p(out), n),
                    "vx_chunk_noise")
@@ -763,9 +763,10 @@ class Server:
             if n.value < 1024:
                 return out
 
-    def lookup(self, coord, edge: int):
+    def lookup(self, coord, edge: int, terrain_id: int):
         ch, me = P(), P()
-        st = self.ctx.L.vx_server_lookup(self.h, Coord(*coord), edge, C.byref(ch), C.byref(me))
+        st = self.ctx.L.vx_server_lookup(self.h, Coord(*coord), edge, terrain_id, C.byref(ch),
+                                         C.byref(me))
         if st == ERR_NOT_READY:
             return None
         self.ctx.check(st, "vx_server_lookup")
