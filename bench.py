@@ -144,7 +144,8 @@ def frame_leg(ctx, faces_per_chunk: float, frames: int = 20):
     B.run()
     new_faces = B.wait()
     frame_ms = {}
-    for form, mode in (("staged", V.BATCH_STAGED), ("fused", V.BATCH_FUSED)):
+    for form, mode in (("staged", V.BATCH_STAGED), ("fused", V.BATCH_FUSED),
+                       ("pipelined", V.BATCH_PIPELINED)):
         # fused: new chunks and the cull in one persistent launch (config 5's form)
         B.set_mode(mode)
         B.set_cull(camera(), cbuf, len(rc), EDGE, mbuf)
@@ -413,8 +414,8 @@ def main():
     ap.add_argument("--no-world", action="store_true", help="skip the tick-by-tick world leg")
     ap.add_argument("--no-two", action="store_true",
                     help="skip the two-batches-in-flight leg (two streams)")
-    ap.add_argument("--pmc-run", choices=["fused", "staged", "instances", "config3", "frame",
-                                          "server"], default=None,
+    ap.add_argument("--pmc-run", choices=["fused", "staged", "pipelined", "instances", "config3",
+                                          "frame", "server"], default=None,
                     help="profiler run of one workload, no timing: the config-2 batch in one "
                          "form (vertices+indices, or InstanceRaw staged), the config-3 ragged "
                          "batch, the config-5 frame, or server flights at 5000 req/s; then a "
@@ -430,10 +431,13 @@ def main():
     if args.pmc_run:
         t, coords, nb = workload(ctx)
         faces = None
-        if args.pmc_run in ("fused", "staged", "instances"):
+        if args.pmc_run in ("fused", "staged", "pipelined", "instances"):
             fmt = V.FMT_INSTANCES if args.pmc_run == "instances" else V.FMT_VERTICES
             b = V.Batch(ctx, coords, EDGE, ctx.terrain(t), nb, formats=fmt)
-            b.set_mode(V.BATCH_FUSED if args.pmc_run == "fused" else V.BATCH_STAGED)
+            b.set_mode({"fused": V.BATCH_FUSED, "pipelined": V.BATCH_PIPELINED}.get(
+                args.pmc_run, V.BATCH_STAGED))
+            if args.pmc_run == "pipelined":
+                b.set_grid(4 * 256)
         elif args.pmc_run == "config3":
             b, _ = config3_batch(ctx)
             b.set_mode(V.BATCH_STAGED)
@@ -492,20 +496,27 @@ def main():
     emit_bytes = n * (N3 // 8 + 4 * N2 // 8 + 4 + 8) + 104 * total_faces
     fused_bytes = fill_bytes + 104 * total_faces + n * 12       # one pass: writes only
 
-    # the other form, for comparison (same outputs)
-    other = None
+    # every other form, for comparison (same outputs): timed like `value`,
+    # kernels timed with HIP events
+    names_of = {V.BATCH_STAGED: "staged", V.BATCH_FUSED: "fused", V.BATCH_PIPELINED: "pipelined"}
+    form_ms = {mode: chosen_ms}
+    other = []
     reps2 = max(5, args.steps // 2)
-    b.set_mode(V.BATCH_STAGED if mode == V.BATCH_FUSED else V.BATCH_FUSED)
-    b.run()
-    assert b.wait() == total_faces
-    oe = timed(reps2, 1)
-    other_ms = b.profile(args.profile_reps)
-    other = {"form": "staged" if mode == V.BATCH_FUSED else "fused",
-             "value": round(n * reps2 / oe, 1), "unit": "chunks/s",
-             "ms_per_step": round(oe * 1e3 / reps2, 4)}
+    for m in (V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED):
+        if m == mode:
+            continue
+        b.set_mode(m)
+        if m == V.BATCH_PIPELINED:
+            b.set_grid(4 * 256)
+        b.run()
+        assert b.wait() == total_faces
+        oe = timed(reps2, 1)
+        form_ms[m] = b.profile(args.profile_reps)
+        other.append({"form": names_of[m], "value": round(n * reps2 / oe, 1), "unit": "chunks/s",
+                      "ms_per_step": round(oe * 1e3 / reps2, 4)})
+        b.set_grid(0)
     b.set_mode(V.BATCH_AUTO)
-    stage_ms = other_ms if mode == V.BATCH_FUSED else chosen_ms
-    fused_ms = chosen_ms if mode == V.BATCH_FUSED else other_ms
+    stage_ms = form_ms[V.BATCH_STAGED]
     names = ["fill", "count", "scan", "emit"]
     log("[bench] staged ms: " + ", ".join(f"{k}={v:.4f}" for k, v in zip(names, stage_ms[:4])))
     stages = {
@@ -519,15 +530,23 @@ def main():
         "algorithmic_bytes": {"fill": fill_bytes, "emit": emit_bytes, "fused": fused_bytes},
         "algorithmic_flops": fill_flops,
     }
-    k_ms = float(fused_ms[0])
-    f_tf = fill_flops / (k_ms * 1e-3) / 1e12
-    f_gbs = fused_bytes / (k_ms * 1e-3) / 1e9
-    stages["fused_kernel"] = {"ms": round(k_ms, 4), "TFLOP/s": round(f_tf, 2), "GB/s": round(f_gbs, 1)}
-    log(f"[bench] fused kernel {k_ms:.4f} ms: {f_tf:.1f} TFLOP/s, {f_gbs:.0f} GB/s")
-    if mode == V.BATCH_FUSED:
-        roof = {"kernel": "fused_kernel<32> (noise+fill+cull+scan+emit)", "bound": "valu",
+    one_launch = {}
+    for m, label in ((V.BATCH_FUSED, "fused_kernel"), (V.BATCH_PIPELINED, "pipe_kernel")):
+        k_ms = float(form_ms[m][0])
+        one_launch[m] = (k_ms, fill_flops / (k_ms * 1e-3) / 1e12, fused_bytes / (k_ms * 1e-3) / 1e9)
+        stages[label] = {"ms": round(k_ms, 4), "TFLOP/s": round(one_launch[m][1], 2),
+                         "GB/s": round(one_launch[m][2], 1)}
+        log(f"[bench] {label} {k_ms:.4f} ms: {one_launch[m][1]:.1f} TFLOP/s, "
+            f"{one_launch[m][2]:.0f} GB/s")
+    if mode in one_launch:
+        k_ms, f_tf, f_gbs = one_launch[mode]
+        kname = ("fused_kernel<32> (noise+fill+cull+scan+emit)" if mode == V.BATCH_FUSED else
+                 "pipe_kernel<32> (fill tasks and cull+scan+emit tasks, one launch)")
+        kshort = "fused_kernel<32>" if mode == V.BATCH_FUSED else "pipe_kernel<32>"
+        roof = {"kernel": kname, "bound": "valu",
                 "achieved": round(f_tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(f_tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(f_tf / FP32_PEAK_TFLOPS, 4),
+                "traffic": measured_traffic(kshort, total_faces),
                 "ms": round(k_ms, 4), "hbm_GB/s": round(f_gbs, 1),
                 "hbm_frac": round(f_gbs / HBM_PEAK_GBS, 4)}
     elif stage_ms[0] >= stage_ms[3]:
@@ -628,13 +647,13 @@ def main():
                    "chunks_per_step": n, "edge": EDGE, "faces_per_step": total_faces},
         "roofline": roof,
         "stages": stages,
-        "form": "fused" if mode == V.BATCH_FUSED else "staged",
+        "form": names_of.get(mode, str(mode)),
         "autotune": tune,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "two_batches_in_flight": two,
         "instances_format": inst,
-        "other_form_gpu": other,
+        "other_forms_gpu": other,
         "full_frame": frame,
         "other_configs": configs,
         "serving": serving,

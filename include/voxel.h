@@ -239,7 +239,6 @@ int vx_frustum_draw_list(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords
                          const uint32_t *faces_dev, const uint64_t *first_dev, int32_t n,
                          int32_t edge, uint32_t mode, vx_draw_indexed *args_dev,
                          uint32_t *count_dev);
-
 /* ---- Batches: many Chunk::new + create_mesh jobs in one launch sequence --- */
 /* One request: chunk coord, edge (16, 32 or 64), terrain, and the neighbour
  * snapshot as indices into the same batch (-1 = absent). */
@@ -275,16 +274,24 @@ int vx_batch_destroy(vx_batch *batch);
  *                    single-pass device-wide scan.  Eligible when every in-batch
  *                    neighbour is the adjacent chunk with the same terrain.
  *   VX_BATCH_STAGED  four kernels: fill -> count -> scan -> emit
- * Both forms produce identical bytes. */
+ *   VX_BATCH_PIPELINED  one persistent kernel whose workgroups each fill their
+ *                    next chunk before meshing their previous one, so noise
+ *                    (VALU) and emission (HBM stores) run side by side on
+ *                    every CU; neighbour planes come from the neighbours'
+ *                    fills (recomputed only if one has not finished).
+ *                    Eligible as VX_BATCH_FUSED, for edges up to 32.
+ * Every form produces identical bytes. */
 #define VX_BATCH_AUTO 0
 #define VX_BATCH_FUSED 1
 #define VX_BATCH_STAGED 2
+#define VX_BATCH_PIPELINED 3
 int vx_batch_set_mode(vx_batch *batch, int32_t mode);
-/* Workgroups of the fused launch in VX_BATCH_FUSED (0 = one per chunk;
- * fewer = persistent workgroups that keep taking chunk tickets).  The staged
- * form ignores it. */
+/* Workgroups of the persistent launch in VX_BATCH_FUSED / VX_BATCH_PIPELINED
+ * (0 = one per chunk; fewer = persistent workgroups that keep taking chunk
+ * tickets).  The staged form ignores it. */
 int vx_batch_set_grid(vx_batch *batch, int32_t grid);
-/* The form the next run uses (VX_BATCH_FUSED or VX_BATCH_STAGED). */
+/* The form the next run uses (VX_BATCH_FUSED, VX_BATCH_STAGED or
+ * VX_BATCH_PIPELINED). */
 int vx_batch_get_mode(vx_batch *batch, int32_t *mode);
 /* Queues fill -> count -> scan -> emit on the context stream (replayed from a
  * captured hipGraph after the first run).  Asynchronous. */
@@ -307,19 +314,22 @@ int vx_batch_draw_list(vx_ctx *ctx, vx_batch *batch, const vx_frustum *f, uint32
                        vx_draw_indexed *args_dev, uint32_t *count_dev);
 /* Times `reps` replays of each stage with HIP events on the context stream.
  * Staged form: ms[0..3] = average fill, count, scan, emit kernel time;
- * fused form: ms[0] = fused kernel (with its control reset), ms[1..3] = 0;
+ * fused / pipelined form: ms[0] = the kernel (with its control reset),
+ * ms[1..3] = 0;
  * ms[4] = whole run replayed from the graph. */
 int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
 /* Autotuner: times every candidate execution form and launch shape of this
  * batch (staged; fused with one workgroup per chunk; fused with 2..4
- * persistent workgroups per CU) over `reps` graph replays each, keeps the
+ * persistent workgroups per CU; pipelined with one workgroup per chunk and
+ * with 3 or 4 persistent workgroups per CU) over `reps` graph replays each,
+ * keeps the
  * fastest for the batch, and caches the choice for batches of the same shape
  * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
  * VX_TUNE_CACHE set to a file path the cache persists across processes.
  * Batches created later in VX_BATCH_AUTO use a cached choice. */
 #define VX_TUNE_MAX_CANDIDATES 8
 typedef struct vx_tune_result {
-    int32_t mode;  /* chosen VX_BATCH_FUSED / VX_BATCH_STAGED */
+    int32_t mode;  /* chosen VX_BATCH_FUSED / VX_BATCH_STAGED / VX_BATCH_PIPELINED */
     int32_t grid;  /* chosen launch shape (vx_batch_set_grid) */
     float best_ms;
     int32_t n_candidates;

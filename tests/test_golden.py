@@ -80,15 +80,18 @@ def test_oracle_frustum_matches_golden(oracle_mod):
 
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode,grid", [(V.BATCH_FUSED, 0), (V.BATCH_FUSED, 2), (V.BATCH_STAGED, 0)])
+@pytest.mark.parametrize("mode,grid", [(V.BATCH_FUSED, 0), (V.BATCH_FUSED, 2), (V.BATCH_STAGED, 0),
+                                       (V.BATCH_PIPELINED, 0), (V.BATCH_PIPELINED, 2)])
 @pytest.mark.parametrize("name", sorted(GOLDEN["batches"]))
 def test_library_batches_match_golden(ctx, name, mode, grid):
     g = GOLDEN["batches"][name]
     tid = ctx.terrain(lib_terrain(g["terrain"]))
     b = V.Batch(ctx, g["coords"], g["edges"], tid, g["neighbours"],
                 formats=V.FMT_VERTICES | V.FMT_INSTANCES)
-    if mode == V.BATCH_FUSED and GOLDEN["terrains"][g["terrain"]]["kind"] == V.TERRAIN_REFERENCE:
-        with pytest.raises(V.VoxelError):  # this terrain runs staged only
+    ref_rule = GOLDEN["terrains"][g["terrain"]]["kind"] == V.TERRAIN_REFERENCE
+    if (mode == V.BATCH_FUSED and ref_rule) or \
+            (mode == V.BATCH_PIPELINED and (ref_rule or max(g["edges"]) > 32)):
+        with pytest.raises(V.VoxelError):  # this batch is not eligible for the form
             b.set_mode(mode)
         b.close()
         return

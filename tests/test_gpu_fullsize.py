@@ -30,7 +30,7 @@ def check_sample(O, b, t, coords, edges, nb, idx, verts, idxs):
         assert (idxs[6 * first:6 * (first + cnt)] == ix).all(), i
 
 
-@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED])
+@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED])
 def test_config2_full_grid(ctx, oracle_mod, mode):
     """4096 chunks of 32^3 (64 x 64 grid, seed 7): every chunk's face count
     equals the oracle's, bases are the exclusive prefix in request order, 16
@@ -43,6 +43,8 @@ def test_config2_full_grid(ctx, oracle_mod, mode):
     nb = V.grid_neighbours(64, 64)
     b = V.Batch(ctx, coords, 32, ctx.terrain(C2), nb, formats=V.FMT_VERTICES)
     b.set_mode(mode)
+    if mode == V.BATCH_PIPELINED:
+        b.set_grid(4 * 256)  # persistent: four workgroups per CU
     b.run()
     total = b.wait()
     t = oterrain(O, C2)
@@ -166,5 +168,48 @@ def test_config5_full_resident_grid_and_cull(ctx, oracle_mod):
         got_m = mb.download(np.uint8, len(coords))
         ref_m = O.frustum_contains(f.planes, coords, 32)
         assert (got_m == ref_m).all() and got_m.sum() > 0
+    # The frame: 512 new chunks (columns x = 256, 257) meshed beside the
+    # resident grid -- their -x neighbours are the resident column x = 255,
+    # +-z in the same batch -- with the cull of all 65,536 resident AABBs in
+    # the same run, in every form; every new chunk byte for byte.
+    say("config 5 frame at full size")
+    t5 = oterrain(O, C5)
+    ext = [b.chunk((side - 1) * side + z) for z in range(side)]
+    nc = [(side + dx, 0, z) for dx in range(2) for z in range(side)]
+    fnb = []
+    for dx in range(2):
+        for z in range(side):
+            i = dx * side + z
+            fnb.append([(-2 - z) if dx == 0 else (i - side), (i + side) if dx == 0 else -1,
+                        i - 1 if z > 0 else -1, i + 1 if z < side - 1 else -1])
+    ids = {c: O.chunk_fill(t5, *c, 32, fast=True) for c in nc}
+    ids.update({(side - 1, 0, z): O.chunk_fill(t5, side - 1, 0, z, 32, fast=True)
+                for z in range(side)})
+    eye = np.array([8000.0, 60.0, 4000.0])
+    f = V.Frustum(T.colmajor(T.perspective(np.radians(70), 16 / 9, 0.1, 1024) @
+                             T.look_at(eye, eye + np.array([-1.0, -0.2, 0.3]))))
+    ref_m = O.frustum_contains(f.planes, coords, 32)
+    for mode in (V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED):
+        B = V.Batch(ctx, nc, 32, ctx.terrain(C5), fnb, formats=V.FMT_VERTICES, ext=ext)
+        B.set_mode(mode)
+        B.set_cull(f, cb, len(coords), 32, mb)
+        B.run()
+        ftot = B.wait()
+        assert (mb.download(np.uint8, len(coords)) == ref_m).all(), mode
+        verts, idxs = B.mesh(V.FMT_VERTICES), B.mesh(V.FMT_INDICES)
+        run = 0
+        for i, c in enumerate(nc):
+            adj = [ids.get((c[0] - 1, 0, c[2])), ids.get((c[0] + 1, 0, c[2])),
+                   ids.get((c[0], 0, c[2] - 1)), ids.get((c[0], 0, c[2] + 1))]
+            _, _, vert, ix = O.mesh_buffers(ids[c], 32, (c[0] * 32, 0, c[2] * 32), adj)
+            first, cnt = B.chunk_faces(i)
+            assert (first, cnt) == (run, len(ix) // 6), (mode, i)
+            assert (bits(verts[4 * first:4 * (first + cnt)]) == bits(vert)).all(), (mode, i)
+            assert (idxs[6 * first:6 * (first + cnt)] == ix).all(), (mode, i)
+            run += cnt
+        assert run == ftot
+        B.close()
+    for c in ext:
+        c.release()
     cb.free()
     mb.free()

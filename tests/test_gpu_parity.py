@@ -222,8 +222,13 @@ def test_generated_chunk_mesh_with_generated_neighbours(ctx, oracle_mod):
 
 # ------------------------------------------------------------------ batches
 def apply_mode(b, mode):
-    """mode: a VX_BATCH_* value or (mode, grid), grid = fused workgroups."""
+    """mode: a VX_BATCH_* value or (mode, grid), grid = persistent workgroups."""
     m, g = mode if isinstance(mode, tuple) else (mode, 0)
+    if m == V.BATCH_PIPELINED and int(np.max(b.edges)) > 32:
+        with pytest.raises(V.VoxelError) as e:
+            b.set_mode(m)
+        assert e.value.status == V.ERR_INVALID
+        pytest.skip("the pipelined form takes edges up to 32 (refused as specified)")
     b.set_mode(m)
     b.set_grid(g)
 
@@ -261,10 +266,15 @@ def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES 
 
 
 # fused with one workgroup per chunk; fused with 3 persistent workgroups that
-# take every chunk ticket in turn; staged
+# take every chunk ticket in turn; staged; pipelined with one workgroup per
+# chunk, with 3 persistent workgroups, and with a single workgroup (which
+# must fill ahead until each mesh is ready, then drain the meshes)
 MODES = [pytest.param((V.BATCH_FUSED, 0), id="fused"),
          pytest.param((V.BATCH_FUSED, 3), id="fused_persistent3"),
-         pytest.param((V.BATCH_STAGED, 0), id="staged")]
+         pytest.param((V.BATCH_STAGED, 0), id="staged"),
+         pytest.param((V.BATCH_PIPELINED, 0), id="pipelined"),
+         pytest.param((V.BATCH_PIPELINED, 3), id="pipelined_persistent3"),
+         pytest.param((V.BATCH_PIPELINED, 1), id="pipelined_single")]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -277,7 +287,9 @@ def test_batch_config1_grid(ctx, oracle_mod, mode):
 
 
 @pytest.mark.parametrize("mode,per_chunk", [(V.BATCH_FUSED, False), (V.BATCH_STAGED, False),
-                                            (V.BATCH_STAGED, True), ((V.BATCH_FUSED, 1), False)])
+                                            (V.BATCH_STAGED, True), ((V.BATCH_FUSED, 1), False),
+                                            (V.BATCH_PIPELINED, False),
+                                            ((V.BATCH_PIPELINED, 7), False)])
 def test_batch_config2_subgrid(ctx, oracle_mod, mode, per_chunk):
     """Config 2 terrain on an 8x8 sub-grid of 32^3 chunks."""
     say("batch C2 8x8", mode, per_chunk)

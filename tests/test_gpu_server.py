@@ -123,7 +123,7 @@ def test_two_terrains_same_coordinate(ctx, oracle_mod):
         s.poll()
         s.submit([((3, 0, 5), 32, tb, V.NB_RESIDENT), ((4, 0, 5), 32, tb, V.NB_RESIDENT)])
         s.drain()
-        comps = {(tuple(c.coord), c.edge): c for c in s.poll()}
+        comps = {((c.coord.x, c.coord.y, c.coord.z), c.edge): c for c in s.poll()}
         assert list(comps[((3, 0, 5), 32)].nb_present) == [0, 0, 1, 0]  # b's (3,0,4)
         assert list(comps[((4, 0, 5), 32)].nb_present) == [0, 0, 0, 0]  # a's (4,0,4) is not b's
         for tid, ot in ((ta, oa), (tb, ob)):

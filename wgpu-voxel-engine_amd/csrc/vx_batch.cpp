@@ -159,15 +159,24 @@ hipError_t enqueue_emit(vx_batch *b) {
                        b->ctx->tables_dev, b->overflow, b->ctx->stream);
 }
 
-// VX_BATCH_AUTO runs the staged form unless the autotuner chose the fused
-// kernel for this batch shape: measured on this part the staged graph is as
-// fast or faster from 64 chunks (0.20 vs 0.21 ms) through a 512-chunk frame
-// with its cull job (0.23 vs 0.24 ms) to 4096 chunks (0.76 vs 1.1 ms).
-bool use_fused(const vx_batch *b) {
-    if (!b->fused_ok || b->mode == VX_BATCH_STAGED) return false;
-    if (b->mode == VX_BATCH_AUTO) return b->tuned && b->tuned_mode == VX_BATCH_FUSED;
-    return true;
+// The form a run takes.  VX_BATCH_AUTO runs the staged form unless the
+// autotuner chose a persistent form (fused or pipelined) for this batch
+// shape: the plain fused kernel measured as slow or slower than the staged
+// graph at every size, from 64 chunks (0.20 vs 0.21 ms) through a 512-chunk
+// frame with its cull job (0.23 vs 0.24 ms) to 4096 chunks (0.76 vs 1.1 ms).
+int form_of(const vx_batch *b) {
+    if (!b->fused_ok || b->mode == VX_BATCH_STAGED) return VX_BATCH_STAGED;
+    int m = b->mode;
+    if (m == VX_BATCH_AUTO) {
+        if (!b->tuned) return VX_BATCH_STAGED;
+        m = b->tuned_mode;
+    }
+    if (m == VX_BATCH_PIPELINED && !b->pipe_ok) return VX_BATCH_STAGED;
+    return m == VX_BATCH_FUSED || m == VX_BATCH_PIPELINED ? m : VX_BATCH_STAGED;
 }
+
+// A persistent form (one launch per run, the fused or the pipelined kernel).
+bool use_fused(const vx_batch *b) { return form_of(b) != VX_BATCH_STAGED; }
 
 int32_t fused_grid_of(const vx_batch *b) {
     return b->mode == VX_BATCH_AUTO && b->tuned ? b->tuned_grid : b->grid;
@@ -175,6 +184,11 @@ int32_t fused_grid_of(const vx_batch *b) {
 
 hipError_t enqueue_fused(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
+    if (form_of(b) == VX_BATCH_PIPELINED)
+        return launch_pipe(b->fused_dev, b->n, b->max_edge, b->pipe_lag, ctx->terrains_dev,
+                           ctx->lut_dev,
+                           ctx->tables_dev, b->fctl_ptrs, b->out, b->cull, fused_grid_of(b),
+                           ctx->stream);
     return launch_fused(b->fused_dev, b->n, b->max_edge, ctx->terrains_dev, ctx->lut_dev,
                         ctx->tables_dev, b->fctl_ptrs, b->out, b->cull, fused_grid_of(b),
                         ctx->stream);
@@ -422,23 +436,33 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
                 break;
             }
         }
+    // pipelined form: edges up to 32 (its LDS holds four workgroups per CU)
+    b->pipe_ok = b->fused_ok && b->max_edge <= 32;
     std::vector<FusedWork> fu;
     if (b->fused_ok) {
         fu.resize(n);
         for (int i = 0; i < n; i++) {
             fu[i].f = fw[i];
             fu[i].m = mw[i];
-            for (int k = 0; k < 4; k++)
+            for (int k = 0; k < 4; k++) {
                 fu[i].halo[k] = reqs[i].nb[k] >= 0    ? kHaloRecompute
                                 : reqs[i].nb[k] <= -2 ? kHaloResident
                                                       : kHaloAbsent;
+                fu[i].nbi[k] = reqs[i].nb[k] >= 0 ? reqs[i].nb[k] : -1;
+            }
         }
         if ((st = ctx->dev_alloc((size_t)n * sizeof(FusedWork), (void **)&b->fused_dev)))
             return fail(st);
-        if ((st = ctx->dev_alloc(256 + (size_t)n * 8, &b->fctl))) return fail(st);
+        // ticket | error | ... | status[n] (u64) | filled[n] (u32)
+        if ((st = ctx->dev_alloc(256 + (size_t)n * 12, &b->fctl))) return fail(st);
         b->fctl_ptrs.ticket = (uint32_t *)b->fctl;
         b->fctl_ptrs.error = (uint32_t *)b->fctl + 1;
         b->fctl_ptrs.status = (uint64_t *)((char *)b->fctl + 256);
+        b->fctl_ptrs.filled = (uint32_t *)(b->fctl_ptrs.status + n);
+        b->fctl_ptrs.mesh_ticket = (uint32_t *)b->fctl + 2;
+        // the farthest in-batch neighbour ahead of a chunk, in batch order
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 4; k++) b->pipe_lag = std::max(b->pipe_lag, reqs[i].nb[k] - i);
         b->fctl_ptrs.counts = b->counts;
         b->fctl_ptrs.bases = b->bases;
         b->fctl_ptrs.total = b->total;
@@ -481,7 +505,9 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
         std::lock_guard<std::mutex> tg(g_tune_mu);
         tune_load();
         auto it = g_tune.find(tune_key(b));
-        if (it != g_tune.end() && (it->second.mode != VX_BATCH_FUSED || b->fused_ok)) {
+        const int m = it != g_tune.end() ? it->second.mode : VX_BATCH_STAGED;
+        if (it != g_tune.end() && (m != VX_BATCH_FUSED || b->fused_ok) &&
+            (m != VX_BATCH_PIPELINED || b->pipe_ok)) {
             b->tuned = true;
             b->tuned_mode = it->second.mode;
             b->tuned_grid = it->second.grid;
@@ -500,9 +526,10 @@ extern "C" int vx_batch_destroy(vx_batch *b) {
 }
 
 extern "C" int vx_batch_set_mode(vx_batch *b, int32_t mode) {
-    if (!b || mode < VX_BATCH_AUTO || mode > VX_BATCH_STAGED) return VX_ERR_INVALID;
+    if (!b || mode < VX_BATCH_AUTO || mode > VX_BATCH_PIPELINED) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     if (mode == VX_BATCH_FUSED && !b->fused_ok) return VX_ERR_INVALID;
+    if (mode == VX_BATCH_PIPELINED && !b->pipe_ok) return VX_ERR_INVALID;
     if (mode != b->mode) {
         (void)hipStreamSynchronize(b->ctx->stream);
         drop_graph(b);
@@ -524,7 +551,7 @@ extern "C" int vx_batch_set_grid(vx_batch *b, int32_t grid) {
 
 extern "C" int vx_batch_get_mode(vx_batch *b, int32_t *mode) {
     if (!b || !mode) return VX_ERR_INVALID;
-    *mode = use_fused(b) ? VX_BATCH_FUSED : VX_BATCH_STAGED;
+    *mode = form_of(b);
     return VX_OK;
 }
 
@@ -776,6 +803,11 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
         cands.push_back({VX_BATCH_FUSED, 0});
         for (int per = 2; per <= 4; per++)
             if (cus * per < b->n) cands.push_back({VX_BATCH_FUSED, cus * per});
+    }
+    if (b->pipe_ok) {  // persistent workgroups that each fill one chunk ahead
+        cands.push_back({VX_BATCH_PIPELINED, 0});
+        for (int per = 3; per <= 4; per++)
+            if (cus * per < b->n) cands.push_back({VX_BATCH_PIPELINED, cus * per});
     }
     VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a queued replay may still run
     hipEvent_t e0, e1;

@@ -153,6 +153,8 @@ struct vx_batch {
     bool tuned = false;             // an autotuned choice applies in VX_BATCH_AUTO
     int32_t tuned_mode = 0, tuned_grid = 0;
     bool fused_ok = false;          // fused form eligible
+    bool pipe_ok = false;           // pipelined form eligible (fused_ok, edges <= 32)
+    int32_t pipe_lag = 0;           // farthest in-batch neighbour index ahead of a chunk
     vx::FusedWork *fused_dev = nullptr;
     void *fctl = nullptr;           // ticket | error | status[n]
     vx::FusedCtl fctl_ptrs{};

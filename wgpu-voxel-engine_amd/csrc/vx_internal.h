@@ -12,6 +12,8 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kMaxOctaves = 8;     // fBm octaves supported by the noise tables
 constexpr int kDescTile = 4096;    // face descriptors staged in LDS per emit tile
 constexpr int kEmitGrid = 1280;    // emit workgroups that fill the GPU (5 per CU)
+constexpr int kEmitBlock64 = 512;  // threads per emit workgroup for 64^3 batches
+constexpr int kEmitGrid64 = 512;   // those workgroups that fill the GPU (2 per CU)
 constexpr int kFillGrid = 1024;    // fill workgroups that fill the GPU (4 per CU)
 
 // Terrain kinds (mirror of vx_terrain.kind in include/voxel.h).
@@ -107,6 +109,7 @@ struct FusedWork {
     FillWork f;
     MeshWork m;
     uint32_t halo[4];
+    int32_t nbi[4];  // batch index of each in-batch (kHaloRecompute) neighbour, else -1
 };
 
 // Control words of one fused run; zeroed by the reset kernel.
@@ -118,6 +121,8 @@ struct FusedCtl {
     uint64_t *total;     // faces of the whole batch
     uint32_t *overflow;  // set when a chunk did not fit the arena
     uint32_t *error;     // set when a look-back wait exceeded its bound
+    uint32_t *filled;    // per chunk: its fill has published (pipelined form; may be null)
+    uint32_t *mesh_ticket;  // next chunk to mesh (pipelined form; may be null)
 };
 
 // Frustum job carried by a fused launch: planes in device memory (updated
@@ -173,6 +178,14 @@ hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
                         const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
                         FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
                         hipStream_t s);
+// Pipelined persistent form (edges up to 32): fill and mesh tasks from two
+// ticket counters; mesh m is handed out once fills up to m + lag are out
+// (lag >= the farthest in-batch neighbour index minus the chunk's own).
+// Needs ctl.filled and ctl.mesh_ticket.
+hipError_t launch_pipe(const FusedWork *work, int32_t n, int32_t max_edge, int32_t lag,
+                       const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
+                       FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
+                       hipStream_t s);
 hipError_t launch_cull(const CullJob &job, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);
 hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n, int32_t edge,

@@ -32,7 +32,8 @@ __device__ __forceinline__ int popc(uint64_t v) { return __popcll(v); }
 __device__ __forceinline__ int ctz(uint32_t v) { return __ffs(v) - 1; }
 __device__ __forceinline__ int ctz(uint64_t v) { return __ffsll((unsigned long long)v) - 1; }
 
-// Exclusive scan of one u32 per thread over a kBlock-thread workgroup.
+// Exclusive scan of one u32 per thread over a BLOCK-thread workgroup.
+template <int BLOCK = kBlock>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave,
                                                          uint32_t *total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -46,7 +47,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
     __syncthreads();
     uint32_t wbase = 0, sum = 0;
 #pragma unroll
-    for (int i = 0; i < kBlock / 64; i++) {
+    for (int i = 0; i < BLOCK / 64; i++) {
         uint32_t s = s_wave[i];
         if (i < wave) wbase += s;
         sum += s;
@@ -222,16 +223,18 @@ __device__ __forceinline__ bool solid_of(float sum, const TerrainDev &T, int wy)
 // Work items are (line group, y segment of SEG rows) pairs dealt round-robin
 // to the waves, so that the halo lines do not leave some waves a whole extra
 // line group (N = 32 with four halo lines: 36 items of 16 rows, 9 per wave).
+// g_first > 0 skips the first line groups (halo_only: g_first = N / L, the
+// neighbour planes alone, with the chunk's own rows left untouched).
 template <int N, int MAXN, bool DBG, int SEG, bool REF, class SH>
 __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
-                              float *noise_out) {
+                              float *noise_out, int g_first = 0) {
     constexpr int L = 64 / N;
     constexpr int NSEG = N / SEG;
     static_assert(N % SEG == 0, "segments tile the column");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int G = (N + H.n + L - 1) / L;
     const int octaves = T.octaves;
-    for (int item = wave; item < G * NSEG; item += kBlock / 64) {
+    for (int item = g_first * NSEG + wave; item < G * NSEG; item += kBlock / 64) {
         const int g = item / NSEG;
         const int y0 = (item - g * NSEG) * SEG;
         const int li = g * L + lane / N;
@@ -307,7 +310,7 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
 // the rare rehash.  Octaves are summed in the same order as the oracle.
 template <int N, int MAXN, bool DBG, int SEG, int OCT, bool REF, class SH>
 __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
-                                  SH &S, float *noise_out, int seg_only = -1) {
+                                  SH &S, float *noise_out, int seg_only = -1, int g_first = 0) {
     constexpr int L = 64 / N;
     constexpr int NSEG = N / SEG;
     static_assert(N % SEG == 0, "segments tile the column");
@@ -327,7 +330,7 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
             a = a * T.gain;
         }
     }
-    for (int item = wave; item < G * nseg; item += kBlock / 64) {
+    for (int item = g_first * nseg + wave; item < G * nseg; item += kBlock / 64) {
         const int g = item / nseg;
         const int y0 = (seg_only >= 0 ? seg_only : item - g * nseg) * SEG;
         const int li = g * L + lane / N;
@@ -422,11 +425,11 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
 // to [0, height_max]; voxel solid iff wy <= h.
 template <int N, int MAXN, bool DBG, class SH>
 __device__ void heightmap_lines(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
-                                SH &S, float *noise_out) {
+                                SH &S, float *noise_out, int g_first = 0) {
     constexpr int L = 64 / N;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int G = (N + H.n + L - 1) / L;
-    for (int g = wave; g < G; g += kBlock / 64) {
+    for (int g = g_first + wave; g < G; g += kBlock / 64) {
         const int li = g * L + lane / N;
         int wxi, wzi;
         line_column<N>(li, lane, H, ox, oz, wxi, wzi);
@@ -520,8 +523,10 @@ __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const
 // Writes the layers y_lo <= y < y_hi of a chunk whose occupancy rows for
 // them are in S.occ: occupancy and boundary planes always, u8 ids (all solid
 // voxels = kBlockGrass) when write_ids; meta[0] = solid block id from the
-// workgroup that writes layer 0.
-template <int N, int MAXN, class SH>
+// workgroup that writes layer 0.  COHERENT: the boundary planes, which other
+// workgroups of the same launch read, are written with agent-scope stores
+// (coherent across the XCDs' L2 caches without a cache write-back).
+template <int N, int MAXN, bool COHERENT = false, class SH>
 __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, uint32_t solid_type,
                                     int y_lo = 0, int y_hi = N) {
     constexpr int W = N == 64 ? 2 : 1;
@@ -550,7 +555,10 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
             const int z = pl == 2 ? 0 : N - 1;
             v = occ[(y * N + z) * RS + j];
         }
-        w.planes[i] = v;
+        if (COHERENT)
+            __hip_atomic_store(w.planes + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            w.planes[i] = v;
     }
     if (write_ids) {
         // u8 block ids, 16 voxels per 16-byte store
@@ -773,19 +781,21 @@ __global__ void __launch_bounds__(kBlock) pack_one_kernel(FillWork w) {
 // HBM write rate of contiguous 16-B lanes.
 constexpr int kStageWords = 32 * 20;  // 32 records per wave pass: 2.5 KB
 
-template <int MAXN> struct MeshShared {
+// BLOCK threads per workgroup: kBlock, or 512 for the MAXN = 64 emit, whose
+// 32 KB of u64 rows and 16 KB of row offsets allow only 2 workgroups per CU
+// (8 waves of 512 threads keep 4 waves per SIMD storing instead of 2).
+template <int MAXN, int BLOCK = kBlock> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t desc[kDescTile / 2];
     // first face of every row, for the interleaved descriptor walk (4 KB at
-    // MAXN = 32: 28 KB in all, 5 workgroups per CU; 16 KB at MAXN = 64,
-    // whose 32 KB of u64 rows already hold it to 2)
+    // MAXN = 32: 28 KB in all, 5 workgroups per CU; 16 KB at MAXN = 64)
     uint32_t rowfirst[MAXN >= 32 ? MAXN * MAXN : 1];
     float P[6][2][4][3];
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
-    uint32_t red[kBlock / 64];
-    float4 stage[kBlock / 64][kStageWords / 4];
+    uint32_t red[BLOCK / 64];
+    float4 stage[BLOCK / 64][kStageWords / 4];
 };
 
 // Orders the LDS accesses of one wave: staging written by some lanes is read
@@ -795,18 +805,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int N, int MAXN, class SH>
+template <int N, int MAXN, int BLOCK = kBlock, class SH>
 __device__ __forceinline__ void load_rows(const MeshWork &w, SH &S) {
     using R = typename RowType<MAXN>::T;
     constexpr int W = N == 64 ? 2 : 1;
     const int tid = threadIdx.x;
-    for (int r = tid; r < N * N; r += kBlock) {
+    for (int r = tid; r < N * N; r += BLOCK) {
         R v;
         if (W == 2) v = (R)w.occ[2 * r] | ((R)w.occ[2 * r + 1] << (W == 2 ? 32 : 0));
         else v = (R)w.occ[r];
         S.occ[r] = v;
     }
-    for (int i = tid; i < 4 * N; i += kBlock) {
+    for (int i = tid; i < 4 * N; i += BLOCK) {
         const int k = i / N, y = i - k * N;
         R v = 0;
         if (w.nb[k]) {
@@ -1019,11 +1029,13 @@ __device__ __forceinline__ void walk_row(SH &S, int r, uint32_t f, uint32_t t0, 
 // `parts` workgroups may share a chunk (small batches): part p emits tiles
 // p, p + parts, ... and the matching share of the index stores; each part
 // repeats the row scan.
-template <int N, int MAXN, bool INTERLEAVE, class SH>
+template <int N, int MAXN, bool INTERLEAVE, int BLOCK = kBlock, class SH>
 __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
                           SH &S, int part = 0, int parts = 1) {
     using R = typename RowType<MAXN>::T;
-    constexpr int RPT = (N * N) / kBlock;
+    // rows per thread (a 16^3 chunk in a 512-thread workgroup: half the
+    // threads own one row, the others none)
+    constexpr int RPT = (N * N + BLOCK - 1) / BLOCK;
     const int tid = threadIdx.x;
     const bool want_v = (out.formats & kFmtVertices) != 0;
     const bool want_i = (out.formats & kFmtInstances) != 0;
@@ -1032,7 +1044,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         // drain while the rows are scanned and the face descriptors built.
         // quad::INDICES = [0,2,1,3,1,2] + 4*local face; two indices per store
         uint2 *dst = reinterpret_cast<uint2 *>(out.indices + base * 6);
-        for (uint32_t q = (uint32_t)(part * kBlock + tid); q < 3 * F; q += (uint32_t)(parts * kBlock)) {
+        for (uint32_t q = (uint32_t)(part * BLOCK + tid); q < 3 * F; q += (uint32_t)(parts * BLOCK)) {
             const uint32_t face = q / 3, part = q - face * 3;
             const uint32_t b4 = 4 * face;
             uint2 v;
@@ -1049,21 +1061,23 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
     for (int i = 0; i < RPT; i++) {
         R m[6];
         const int r = r0 + i;
-        face_masks<N, MAXN>(S, r / N, r % N, m);
         uint32_t c = 0;
+        if (N * N % BLOCK == 0 || r < N * N) {
+            face_masks<N, MAXN>(S, r / N, r % N, m);
 #pragma unroll
-        for (int d = 0; d < 6; d++) c += popc(m[d]);
+            for (int d = 0; d < 6; d++) c += popc(m[d]);
+        }
         rc[i] = c;
         mine += c;
     }
     uint32_t total;
-    const uint32_t first = block_exclusive_scan(mine, S.red, &total);
+    const uint32_t first = block_exclusive_scan<BLOCK>(mine, S.red, &total);
     (void)total;  // equals F (count kernel); F is authoritative for sizing
     if constexpr (INTERLEAVE) {
         uint32_t f = first;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
-            S.rowfirst[r0 + i] = f;
+            if (N * N % BLOCK == 0 || r0 + i < N * N) S.rowfirst[r0 + i] = f;
             f += rc[i];
         }
         __syncthreads();
@@ -1074,7 +1088,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         const uint32_t t1 = min(F, t0 + TILE);
         if constexpr (INTERLEAVE) {
 #pragma unroll 1
-            for (int r = tid; r < N * N; r += kBlock) {
+            for (int r = tid; r < N * N; r += BLOCK) {
                 const uint32_t f = S.rowfirst[r];
                 const uint32_t e = r + 1 < N * N ? S.rowfirst[r + 1] : F;
                 if (e > t0 && f < t1) walk_row<N, MAXN>(S, r, f, t0, t1);
@@ -1084,7 +1098,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             if (f < t1 && f + mine > t0) {
 #pragma unroll 1
                 for (int i = 0; i < RPT; i++) {
-                    if (f + rc[i] > t0 && f < t1) walk_row<N, MAXN>(S, r0 + i, f, t0, t1);
+                    if (rc[i] && f + rc[i] > t0 && f < t1) walk_row<N, MAXN>(S, r0 + i, f, t0, t1);
                     f += rc[i];
                 }
             }
@@ -1098,7 +1112,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         const uint32_t nt = t1 - t0;
         float4 *st4 = S.stage[wave];
         uint32_t *stw = reinterpret_cast<uint32_t *>(S.stage[wave]);
-        for (uint32_t q0 = 64u * wave; q0 < nt; q0 += kBlock) {
+        for (uint32_t q0 = 64u * wave; q0 < nt; q0 += BLOCK) {
             const uint32_t q = q0 + lane;
             const bool valid = q < nt;
             const uint32_t nv = min(64u, nt - q0);
@@ -1176,51 +1190,50 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
     }
 }
 
-template <int N, int MAXN>
+template <int N, int MAXN, int BLOCK>
 __device__ void emit_chunk(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
-                           MeshShared<MAXN> &S, int part, int parts) {
-    load_rows<N, MAXN>(w, S);
+                           MeshShared<MAXN, BLOCK> &S, int part, int parts) {
+    load_rows<N, MAXN, BLOCK>(w, S);
     __syncthreads();
-    emit_rows<N, MAXN, (MAXN >= 32)>(w, base, F, out, S, part, parts);
+    emit_rows<N, MAXN, (MAXN >= 32), BLOCK>(w, base, F, out, S, part, parts);
 }
 
-template <int MAXN>
+template <int MAXN, int BLOCK = kBlock>
 __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint32_t F,
                                           const MeshOut &out, const MeshTables *tables,
-                                          uint32_t *overflow, MeshShared<MAXN> &S, int part,
+                                          uint32_t *overflow, MeshShared<MAXN, BLOCK> &S, int part,
                                           int parts) {
     if (base + F > out.capacity) {
         if (threadIdx.x == 0) atomicOr(overflow, 1u);
         return;
     }
-    for (int i = threadIdx.x; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
-    for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
-    for (int i = threadIdx.x; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 2 * 4 * 3; i += BLOCK) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += BLOCK) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 18; i += BLOCK) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     switch (w.edge) {
-    case 16: emit_chunk<16, MAXN>(w, base, F, out, S, part, parts); break;
-    case 32: if (MAXN >= 32) emit_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, base, F, out, S, part, parts); break;
-    case 64: if (MAXN >= 64) emit_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(w, base, F, out, S, part, parts); break;
+    case 16: emit_chunk<16, MAXN, BLOCK>(w, base, F, out, S, part, parts); break;
+    case 32: if (MAXN >= 32) emit_chunk<(MAXN >= 32 ? 32 : 16), MAXN, BLOCK>(w, base, F, out, S, part, parts); break;
+    case 64: if (MAXN >= 64) emit_chunk<(MAXN >= 64 ? 64 : 16), MAXN, BLOCK>(w, base, F, out, S, part, parts); break;
     default: break;
     }
 }
 
-// Register cap: five waves per SIMD (96 VGPRs), matching the five
-// workgroups per CU that the 28 KB of LDS allows at MAXN = 32, so more
-// workgroups overlap their descriptor and store phases.
-template <int MAXN>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) emit_kernel(const MeshWork *__restrict__ work,
-                                                      const int32_t *__restrict__ order,
-                                                      const uint64_t *__restrict__ bases,
-                                                      const uint32_t *__restrict__ counts,
-                                                      MeshOut out,
-                                                      const MeshTables *__restrict__ tables,
-                                                      uint32_t *__restrict__ overflow,
-                                                      int parts, int32_t n) {
-    __shared__ MeshShared<MAXN> S;
+// Register cap: five waves per SIMD (96 VGPRs) with 256 threads, matching
+// the five workgroups per CU that the 28 KB of LDS allows at MAXN = 32, so
+// more workgroups overlap their descriptor and store phases; four (128
+// VGPRs) for the 512-thread MAXN = 64 form, two workgroups per CU.
+template <int MAXN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 512 ? 4 : 5, 8)))
+emit_kernel(const MeshWork *__restrict__ work, const int32_t *__restrict__ order,
+            const uint64_t *__restrict__ bases, const uint32_t *__restrict__ counts, MeshOut out,
+            const MeshTables *__restrict__ tables, uint32_t *__restrict__ overflow, int parts,
+            int32_t n) {
+    __shared__ MeshShared<MAXN, BLOCK> S;
     const int k = (int)blockIdx.x / parts, part = (int)blockIdx.x - k * parts;
     const int item = order ? order[k] : k;
     if (!order_entry_ok(item, n, overflow + 1)) return;
-    emit_item<MAXN>(work[item], bases[item], counts[item], out, tables, overflow, S, part, parts);
+    emit_item<MAXN, BLOCK>(work[item], bases[item], counts[item], out, tables, overflow, S, part,
+                           parts);
 }
 
 template <int MAXN>
@@ -1278,34 +1291,16 @@ __device__ __forceinline__ uint64_t status_load(uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int N, int MAXN>
-__device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int n,
-                            const FusedCtl &ctl, const MeshOut &out, FusedShared<MAXN> &S) {
-    using R = typename RowType<MAXN>::T;
-    constexpr int WW = N == 64 ? 2 : 1;
+// Decoupled look-back (Merrill & Garland 2016) over the chunk tickets: ticket
+// t publishes its face count F, sums its predecessors' counts (64 per round
+// trip, stopping at the nearest inclusive prefix), publishes its inclusive
+// prefix and leaves its first face in S.base.  Waits are only on earlier
+// tickets (held by running workgroups) and bounded (kSpinLimit polls, then
+// ctl.error).  Wave 0 of the workgroup runs it; S.base is valid after the
+// caller's next barrier.
+template <class SH>
+__device__ void publish_and_look_back(const FusedCtl &ctl, int t, int n, uint32_t F, SH &S) {
     const int tid = threadIdx.x;
-    const FillWork &w = W.f;
-    Halo H{0, 0u};
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (W.halo[k] == kHaloRecompute) H.sides |= (uint32_t)k << (2 * H.n++);
-    for (int i = tid; i < 4 * N; i += kBlock) {
-        const int k = i / N, y = i - k * N;
-        if (W.halo[k] == kHaloRecompute) continue;
-        R v = 0;
-        const uint32_t *p = W.m.nb[k];
-        if (W.halo[k] == kHaloResident && p) {
-            if (WW == 2) v = (R)p[2 * y] | ((R)p[2 * y + 1] << (WW == 2 ? 32 : 0));
-            else v = (R)p[y];
-        }
-        S.nb[k][y] = v;
-    }
-    // Halo lines make N + 4 lines: shorter y segments keep the waves even.
-    generate_rows<N, MAXN, false, (N == 64 ? 32 : N / 2), false>(T, w.cx * N, w.cy * N, w.cz * N,
-                                                                 H, S, nullptr);
-    __syncthreads();
-    write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
-    const uint32_t F = count_rows<N, MAXN>(S);
     if (tid < 64) {  // wave 0: publish, then look back 64 predecessors per round trip
         const int lane = tid;
         uint64_t prefix = 0;
@@ -1346,6 +1341,37 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
             S.base = prefix;
         }
     }
+}
+
+template <int N, int MAXN>
+__device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int n,
+                            const FusedCtl &ctl, const MeshOut &out, FusedShared<MAXN> &S) {
+    using R = typename RowType<MAXN>::T;
+    constexpr int WW = N == 64 ? 2 : 1;
+    const int tid = threadIdx.x;
+    const FillWork &w = W.f;
+    Halo H{0, 0u};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (W.halo[k] == kHaloRecompute) H.sides |= (uint32_t)k << (2 * H.n++);
+    for (int i = tid; i < 4 * N; i += kBlock) {
+        const int k = i / N, y = i - k * N;
+        if (W.halo[k] == kHaloRecompute) continue;
+        R v = 0;
+        const uint32_t *p = W.m.nb[k];
+        if (W.halo[k] == kHaloResident && p) {
+            if (WW == 2) v = (R)p[2 * y] | ((R)p[2 * y + 1] << (WW == 2 ? 32 : 0));
+            else v = (R)p[y];
+        }
+        S.nb[k][y] = v;
+    }
+    // Halo lines make N + 4 lines: shorter y segments keep the waves even.
+    generate_rows<N, MAXN, false, (N == 64 ? 32 : N / 2), false>(T, w.cx * N, w.cy * N, w.cz * N,
+                                                                 H, S, nullptr);
+    __syncthreads();
+    write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
+    const uint32_t F = count_rows<N, MAXN>(S);
+    publish_and_look_back(ctl, t, n, F, S);
     __syncthreads();
     const uint64_t base = S.base;
     if (base + F > out.capacity) {
@@ -1436,11 +1462,205 @@ fused_kernel(const FusedWork *__restrict__ work, int32_t n,
 __global__ void __launch_bounds__(kBlock) fused_reset_kernel(FusedCtl ctl, int32_t n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) ctl.status[i] = 0ull;
+    if (i < n && ctl.filled) ctl.filled[i] = 0u;
     if (i == 0) {
         *ctl.ticket = 0u;
+        if (ctl.mesh_ticket) *ctl.mesh_ticket = 0u;
         *ctl.overflow = 0u;
         *ctl.error = 0u;
         *ctl.total = 0ull;
+    }
+}
+
+// ------------------------------------------------- pipelined persistent kernel
+// One launch for the whole batch, like fused_kernel, with fills and meshes
+// as separate tasks taken from two ticket counters:
+//
+//   fill ticket t:  noise -> ids, rows, planes of chunk t; publish filled[t]
+//   mesh ticket m:  rows of m and its neighbours' planes from memory, count,
+//                   look-back for the first face, emit
+//
+// Every loop iteration a workgroup takes a fill ticket and fills it, then
+// takes a mesh ticket m (both by atomic add: a compare-and-swap claim of
+// ready tickets serialised the workgroups and ran 80 times slower).  Mesh m
+// reads the fills of chunks up to m + lag (the farthest in-batch neighbour);
+// until fill tickets that far have all been handed out, the workgroup takes
+// and runs further fill tickets itself rather than wait.  So on every CU some
+// workgroups run the VALU-bound noise while others run the store-bound emit,
+// at the fill's register budget (4 waves per SIMD) and with no halo noise.
+// Waits end: a mesh waits only for fills already handed out (fills never
+// wait) and, in the look-back, for earlier mesh tickets, whose holders are
+// running and themselves only fill or wait on earlier tickets; all waits are
+// bounded (kSpinLimit polls, then ctl.error).  Rows and planes cross
+// workgroups through agent-scope loads and stores (coherent across the XCDs'
+// L2 caches without a cache write-back), the stores completed before the
+// flag is published.
+template <int MAXN> struct PipeShared {
+    int perm[512];
+    float4 lut[8][16];
+    float4 ytab[kMaxOctaves * MAXN];
+    uint32_t ychg[MAXN];
+    typename RowType<MAXN>::T occ[MAXN * MAXN];
+    typename RowType<MAXN>::T nb[4][MAXN];
+    uint32_t desc[kDescTile / 2];
+    uint32_t rowfirst[MAXN * MAXN];  // interleaved descriptor walk
+    float P[6][2][4][3];
+    float Mb[6][2][3][3];
+    uint32_t tex[3][6];
+    uint32_t red[kBlock / 64];
+    float4 stage[kBlock / 64][kStageWords / 4];
+    int item;
+    int mitem;
+    int terrain;
+    uint64_t base;
+};
+
+template <int N, int MAXN>
+__device__ void pipe_fill(const FillWork &w, const TerrainDev &T, uint32_t *filled,
+                          PipeShared<MAXN> &S) {
+    const Halo H{0, 0u};
+    generate_rows<N, MAXN, false, (N < 32 ? N : 32), true>(T, w.cx * N, w.cy * N, w.cz * N, H, S,
+                                                           nullptr);
+    __syncthreads();
+    write_chunk_outputs<N, MAXN, true>(w, S, true, kBlockGrass);
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
+    __syncthreads();                // ... and every thread's
+    if (threadIdx.x == 0) __hip_atomic_store(filled, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits (bounded) until chunk j's fill has published.
+__device__ __forceinline__ void pipe_wait_filled(const FusedCtl &ctl, int j) {
+    uint32_t spins = 0;
+    while (!__hip_atomic_load(ctl.filled + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        if (++spins > kSpinLimit) {
+            atomicOr(ctl.error, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ uint32_t coherent_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int N, int MAXN>
+__device__ void pipe_mesh(const FusedWork &W, int m, int n, const FusedCtl &ctl,
+                          const MeshOut &out, PipeShared<MAXN> &S) {
+    using R = typename RowType<MAXN>::T;
+    constexpr int WW = N == 64 ? 2 : 1;
+    const int tid = threadIdx.x;
+    if (tid == 0) {  // the chunk's own fill and its in-batch neighbours' fills
+        pipe_wait_filled(ctl, m);
+        for (int k = 0; k < 4; k++)
+            if (W.halo[k] == kHaloRecompute) pipe_wait_filled(ctl, W.nbi[k]);
+    }
+    __syncthreads();
+    for (int r = tid; r < N * N; r += kBlock) {
+        R v;
+        if (WW == 2)
+            v = (R)coherent_load(W.f.occ + 2 * r) |
+                ((R)coherent_load(W.f.occ + 2 * r + 1) << (WW == 2 ? 32 : 0));
+        else
+            v = (R)coherent_load(W.f.occ + r);
+        S.occ[r] = v;
+    }
+    for (int i = tid; i < 4 * N; i += kBlock) {
+        const int k = i / N, y = i - k * N;
+        R v = 0;
+        const uint32_t *p = W.m.nb[k];
+        if (W.halo[k] == kHaloRecompute) {  // in this batch: written during this launch
+            if (WW == 2)
+                v = (R)coherent_load(p + 2 * y) |
+                    ((R)coherent_load(p + 2 * y + 1) << (WW == 2 ? 32 : 0));
+            else
+                v = (R)coherent_load(p + y);
+        } else if (W.halo[k] == kHaloResident && p) {  // written before this launch
+            if (WW == 2) v = (R)p[2 * y] | ((R)p[2 * y + 1] << (WW == 2 ? 32 : 0));
+            else v = (R)p[y];
+        }
+        S.nb[k][y] = v;
+    }
+    __syncthreads();
+    const uint32_t F = count_rows<N, MAXN>(S);
+    publish_and_look_back(ctl, m, n, F, S);
+    __syncthreads();
+    const uint64_t base = S.base;
+    if (base + F > out.capacity) {
+        if (tid == 0) atomicOr(ctl.overflow, 1u);
+    } else {
+        emit_rows<N, MAXN, true>(W.m, base, F, out, S);
+    }
+    __syncthreads();
+}
+
+// Register cap: four waves per SIMD, the fill kernel's budget.
+template <int MAXN>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+pipe_kernel(const FusedWork *__restrict__ work, int32_t n, int32_t lag,
+            const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut,
+            const MeshTables *__restrict__ tables, FusedCtl ctl, MeshOut out, CullJob cull) {
+    static_assert(MAXN <= 32, "64^3 chunks take the fused or staged form");
+    __shared__ PipeShared<MAXN> S;
+    const int tid = threadIdx.x;
+    if (cull.n > 0) cull_slice(cull);
+    for (int i = tid; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
+    for (int i = tid; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = tid; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = tid; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
+    if (tid == 0) S.terrain = -1;
+    bool fills_left = true, meshes_left = true;
+    int m = -1;  // mesh ticket held, not yet ready
+    for (;;) {
+        // 1. the next fill
+        if (fills_left) {
+            __syncthreads();
+            if (tid == 0) S.item = (int)atomicAdd(ctl.ticket, 1u);
+            __syncthreads();
+            const int t = __builtin_amdgcn_readfirstlane(S.item);
+            if (t < n) {
+                const FusedWork &W = work[t];
+                const int terrain = W.f.terrain;
+                if (terrain != S.terrain) {
+                    for (int i = tid; i < 512; i += kBlock) S.perm[i] = terrains[terrain].perm[i];
+                    __syncthreads();
+                    if (tid == 0) S.terrain = terrain;
+                }
+                const TerrainDev &T = terrains[terrain];
+                if (W.f.edge == 16) pipe_fill<16, MAXN>(W.f, T, ctl.filled + t, S);
+                else if (MAXN >= 32) pipe_fill<(MAXN >= 32 ? 32 : 16), MAXN>(W.f, T, ctl.filled + t, S);
+            } else {
+                fills_left = false;
+            }
+        }
+        // 2. a mesh ticket, unless one is held (an atomic add: a
+        // compare-and-swap claim of ready tickets serialised the workgroups)
+        __syncthreads();
+        if (tid == 0) {
+            int mm = m;
+            if (mm < 0 && meshes_left) mm = (int)atomicAdd(ctl.mesh_ticket, 1u);
+            // 3. ready once the fills of chunks up to mm + lag are all out
+            const uint32_t f = __hip_atomic_load(ctl.ticket, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            S.mitem = mm;
+            S.item = mm >= 0 && mm < n && (f >= (uint32_t)n || f > (uint32_t)mm + (uint32_t)lag);
+        }
+        __syncthreads();
+        m = __builtin_amdgcn_readfirstlane(S.mitem);
+        const bool ready = __builtin_amdgcn_readfirstlane(S.item) != 0;
+        if (m >= n) {  // every mesh ticket is taken
+            meshes_left = false;
+            m = -1;
+        }
+        if (ready) {
+            const FusedWork &W = work[m];
+            if (W.f.edge == 16) pipe_mesh<16, MAXN>(W, m, n, ctl, out, S);
+            else if (MAXN >= 32) pipe_mesh<(MAXN >= 32 ? 32 : 16), MAXN>(W, m, n, ctl, out, S);
+            m = -1;
+        }
+        // a held mesh that is not ready yet waits for further fills, which
+        // this workgroup takes itself in the next iteration
+        if (!fills_left && !meshes_left && m < 0) break;
     }
 }
 
@@ -1613,17 +1833,20 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
     if (n <= 0) return hipSuccess;
     // small batches: up to 4 workgroups per chunk (8 when the batch holds
     // 64^3 chunks, which reach 12 descriptor tiles of faces), so the grid
-    // still covers about kEmitGrid workgroups and a chunk's tiles are
-    // emitted side by side
-    const int parts =
-        n >= kEmitGrid ? 1 : std::min(max_edge > 32 ? 8 : 4, (kEmitGrid + n - 1) / n);
+    // still covers about a full GPU of workgroups (kEmitGrid of 256 threads,
+    // kEmitGrid64 of 512) and a chunk's tiles are emitted side by side
+    const int full = max_edge > 32 ? kEmitGrid64 : kEmitGrid;
+    const int parts = n >= full ? 1 : std::min(max_edge > 32 ? 8 : 4, (full + n - 1) / n);
     const int grid = n * parts;
     if (max_edge <= 16)
-        emit_kernel<16><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts, n);
+        emit_kernel<16, kBlock><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables,
+                                                        overflow, parts, n);
     else if (max_edge <= 32)
-        emit_kernel<32><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts, n);
+        emit_kernel<32, kBlock><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables,
+                                                        overflow, parts, n);
     else
-        emit_kernel<64><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables, overflow, parts, n);
+        emit_kernel<64, kEmitBlock64><<<grid, kEmitBlock64, 0, s>>>(work, order, bases, counts, out,
+                                                                    tables, overflow, parts, n);
     return hipGetLastError();
 }
 
@@ -1665,6 +1888,22 @@ hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s
     const uint64_t blocks = (n16 + kBlock - 1) / kBlock;
     copy_kernel<<<(unsigned)(blocks < 8192 ? blocks : 8192), kBlock, 0, s>>>(
         (uint4 *)dst, (const uint4 *)src, n16);
+    return hipGetLastError();
+}
+
+hipError_t launch_pipe(const FusedWork *work, int32_t n, int32_t max_edge, int32_t lag,
+                       const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
+                       FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
+                       hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (max_edge > 32 || !ctl.filled || !ctl.mesh_ticket || lag < 0) return hipErrorInvalidValue;
+    fused_reset_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(ctl, n);
+    if (max_edge <= 16)
+        pipe_kernel<16><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, lag, terrains, lut, tables,
+                                                               ctl, out, cull);
+    else
+        pipe_kernel<32><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, lag, terrains, lut, tables,
+                                                               ctl, out, cull);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_CAPACITY, ERR_BUDGET, ERR_NOT_READY,
 TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY, TERRAIN_REFERENCE = range(5)
 FMT_INSTANCES, FMT_VERTICES = 1, 2
 FMT_INDICES = 4  # vx_batch_read_mesh selector for the index buffer
-BATCH_AUTO, BATCH_FUSED, BATCH_STAGED = 0, 1, 2
+BATCH_AUTO, BATCH_FUSED, BATCH_STAGED, BATCH_PIPELINED = 0, 1, 2, 3
 AIR, GRASS, DIRT = 0, 1, 2
 
 INSTANCE_DTYPE = np.dtype([("m", "<f4", (4, 4)), ("tex", "<u4")])  # InstanceRaw, 68 B
@@ -633,7 +633,7 @@ class Batch:
         r = TuneResult()
         self.ctx.check(self.ctx.L.vx_batch_autotune(self.ctx.h, self.h, reps, C.byref(r)),
                        "vx_batch_autotune")
-        names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged"}
+        names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged", BATCH_PIPELINED: "pipelined"}
         return {"mode": names.get(r.mode, r.mode), "grid": r.grid, "best_ms": round(r.best_ms, 4),
                 "candidates": [{"mode": names.get(r.cand_mode[i], r.cand_mode[i]),
                                 "grid": r.cand_grid[i], "ms": round(r.cand_ms[i], 4)}
