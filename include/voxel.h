@@ -239,6 +239,22 @@ int vx_frustum_draw_list(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords
                          const uint32_t *faces_dev, const uint64_t *first_dev, int32_t n,
                          int32_t edge, uint32_t mode, vx_draw_indexed *args_dev,
                          uint32_t *count_dev);
+/* vx_frustum_draw_list that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, a per-frame render loop, harnesses that forbid
+ * allocation).  Same arguments, kernels and results as vx_frustum_draw_list;
+ * its per-workgroup partial counts live in a fixed 16 KiB device array per
+ * GPU, held until the process ends, instead of a context allocation, and it
+ * queues no memset.  n may not exceed VX_DRAW_LIST_NOALLOC_MAX
+ * (VX_ERR_INVALID).  Every call on one GPU uses that GPU's array: calls that
+ * overlap on one GPU (two contexts' streams) race; calls on one context are
+ * ordered by its stream. */
+#define VX_DRAW_LIST_NOALLOC_MAX 4194304
+int vx_frustum_draw_list_noalloc(vx_ctx *ctx, const vx_frustum *f, const int32_t *coords_dev,
+                                 const uint32_t *faces_dev, const uint64_t *first_dev, int32_t n,
+                                 int32_t edge, uint32_t mode, vx_draw_indexed *args_dev,
+                                 uint32_t *count_dev);
+
 /* ---- Batches: many Chunk::new + create_mesh jobs in one launch sequence --- */
 /* One request: chunk coord, edge (16, 32 or 64), terrain, and the neighbour
  * snapshot as indices into the same batch (-1 = absent). */

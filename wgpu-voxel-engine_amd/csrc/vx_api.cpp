@@ -342,6 +342,12 @@ extern "C" int vx_ctx_create(const vx_ctx_desc *desc, vx_ctx **out) {
         vx_ctx_destroy(c);
         return VX_ERR_DEVICE;
     }
+    // device address of this GPU's draw-list scratch (the device is current)
+    if (hipError_t e = draw_scratch_address(dev, &c->draw_scratch); e != hipSuccess) {
+        c->set_error(e, "hipGetSymbolAddress(draw-list scratch)");
+        vx_ctx_destroy(c);
+        return VX_ERR_DEVICE;
+    }
     GradLut lut;
     make_grad_lut(lut);
     MeshTables tabs;
@@ -937,6 +943,36 @@ extern "C" int vx_frustum_draw_list(vx_ctx *ctx, const vx_frustum *f, const int3
     }
     return vx_draw_list_locked(ctx, f, coords_dev, faces_dev, first_dev, n, edge, mode, args_dev,
                                count_dev);
+}
+
+// The allocation-free variant: the per-workgroup counts live in the static
+// scratch array of this context's GPU; nothing is allocated or freed and no
+// memset is queued, so a caller may capture the call into a HIP graph.
+extern "C" int vx_frustum_draw_list_noalloc(vx_ctx *ctx, const vx_frustum *f,
+                                            const int32_t *coords_dev, const uint32_t *faces_dev,
+                                            const uint64_t *first_dev, int32_t n, int32_t edge,
+                                            uint32_t mode, vx_draw_indexed *args_dev,
+                                            uint32_t *count_dev) {
+    if (!ctx || !f || n < 0 || edge <= 0 || !count_dev ||
+        (mode != VX_DRAW_INSTANCED && mode != VX_DRAW_EXPANDED) ||
+        (n && (!coords_dev || !faces_dev || !first_dev || !args_dev)))
+        return VX_ERR_INVALID;
+    if (n > draw_scratch_max_meshes()) {
+        ctx->last_error = "vx_frustum_draw_list_noalloc: more meshes than its scratch serves";
+        return VX_ERR_INVALID;
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!ctx->draw_scratch) return VX_ERR_DEVICE;
+    if (!n) {
+        VX_HIP(ctx, launch_clear_words(count_dev, 2, ctx->stream));
+        return VX_OK;
+    }
+    Planes P;
+    memcpy(P.p, f->planes, sizeof(P.p));
+    VX_HIP(ctx, launch_draw_list(P, coords_dev, faces_dev, first_dev, n, edge, mode,
+                                 ctx->draw_scratch, reinterpret_cast<DrawArgs *>(args_dev),
+                                 count_dev, ctx->stream));
+    return VX_OK;
 }
 
 extern "C" int vx_frustum_mask(vx_ctx *ctx, const vx_frustum *f, const vx_coord *coords, int32_t n,

@@ -84,6 +84,7 @@ struct vx_ctx {
     hipEvent_t free_fence = nullptr;  // recorded on `stream` at every dev_free
     uint32_t *draw_counts = nullptr;  // per-workgroup counts of draw-list passes
     int32_t draw_counts_cap = 0;
+    uint32_t *draw_scratch = nullptr; // this device's static draw-list scratch
 
     void set_error(hipError_t e, const char *what);
     int dev_alloc(size_t n, void **out);

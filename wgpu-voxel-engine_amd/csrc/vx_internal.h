@@ -192,6 +192,14 @@ hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n
                           uint8_t *mask, hipStream_t s);
 // Workgroups (and block_counts entries) a draw list of n meshes uses.
 int32_t draw_blocks(int32_t n);
+// The draw-list scratch array of the allocation-free entry point: device
+// address of `device`'s copy (looked up once per device, with that device
+// current at the first call), and the most meshes it serves.
+constexpr int kMaxDevices = 64;
+hipError_t draw_scratch_address(int device, uint32_t **ptr);
+int32_t draw_scratch_max_meshes();
+// Zeroes n <= 64 words with a kernel (capturable; no memset node).
+hipError_t launch_clear_words(uint32_t *p, int32_t n, hipStream_t s);
 hipError_t launch_draw_list(const Planes &planes, const int32_t *coords, const uint32_t *faces,
                             const uint64_t *first, int32_t n, int32_t edge, uint32_t mode,
                             uint32_t *block_counts, DrawArgs *args, uint32_t *count, hipStream_t s);

@@ -16,6 +16,8 @@
 // exactly the oracle's separate multiply and add.
 #include "vx_internal.h"
 
+#include <mutex>
+
 namespace vx {
 
 // ---------------------------------------------------------------- helpers
@@ -1921,6 +1923,43 @@ hipError_t launch_frustum(const Planes &planes, const int32_t *coords, int32_t n
 }
 
 int32_t draw_blocks(int32_t n) { return (n + kDrawBlock - 1) / kDrawBlock; }
+
+// Scratch of the draw-list entry point that allocates nothing: the count
+// pass's per-workgroup counts, one u32 per kDrawBlock meshes.  One copy per
+// GPU, held until the process ends; every call overwrites the entries it
+// uses before reading them.  Host code never passes the array's name to a
+// kernel: it is a host-side stand-in, and the device address differs per
+// GPU (draw_scratch_address).
+constexpr int32_t kDrawScratchBlocks = 4096;
+static __device__ __align__(256) uint32_t scratch[kDrawScratchBlocks];
+
+int32_t draw_scratch_max_meshes() { return kDrawScratchBlocks * kDrawBlock; }
+
+hipError_t draw_scratch_address(int device, uint32_t **ptr) {
+    static std::mutex mu;
+    static uint32_t *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[device]) {
+        // the caller has made `device` current: this is that GPU's copy
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<uint32_t *>(p);
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+
+__global__ void __launch_bounds__(64) clear_words_kernel(uint32_t *__restrict__ p, int32_t n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+hipError_t launch_clear_words(uint32_t *p, int32_t n, hipStream_t s) {
+    if (n <= 0 || n > 64) return hipErrorInvalidValue;
+    clear_words_kernel<<<1, 64, 0, s>>>(p, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_draw_list(const Planes &planes, const int32_t *coords, const uint32_t *faces,
                             const uint64_t *first, int32_t n, int32_t edge, uint32_t mode,

@@ -170,6 +170,8 @@ _SIGS = {
     "vx_frustum_mask_device": (C.c_int, [P, C.POINTER(FrustumC), P, C.c_int32, C.c_int32, P]),
     "vx_frustum_draw_list": (C.c_int, [P, C.POINTER(FrustumC), P, P, P, C.c_int32, C.c_int32,
                                        C.c_uint32, P, P]),
+    "vx_frustum_draw_list_noalloc": (C.c_int, [P, C.POINTER(FrustumC), P, P, P, C.c_int32,
+                                               C.c_int32, C.c_uint32, P, P]),
     "vx_batch_draw_list": (C.c_int, [P, P, C.POINTER(FrustumC), C.c_uint32, P, P]),
     "vx_batch_create": (C.c_int, [P, P, C.c_int32, C.c_uint32, C.c_uint64, C.POINTER(P)]),
     "vx_batch_create_ext": (C.c_int, [P, P, C.c_int32, P, C.c_int32, C.c_uint32, C.c_uint64,
