@@ -2,7 +2,8 @@
 that guards the arrays the fill and emit launches index through: compiled
 on its own with the host C++ compiler and run on permutations and on the
 corruptions it must refuse (an index out of range, a duplicate, a negative
-entry).  CPU only."""
+entry), and the emit-item form (codes chunk << 2 | slab, four slabs for a
+64^3 chunk) on item lists and their corruptions.  CPU only."""
 import os
 import shutil
 import subprocess
@@ -16,8 +17,21 @@ DRIVER = r"""
 #include <cstdio>
 #include <cstdlib>
 #include "vx_schedule.h"
+#include <cstring>
+#include <vector>
 int main(int argc, char **argv) {
-    // argv[1..]: the schedule; prints 1 if it is a permutation of 0..n-1
+    // argv[1..]: the schedule; prints 1 if it is a permutation of 0..n-1.
+    // "items p0 p1 ... -- c0 c1 ...": items per chunk, then the item codes;
+    // prints 1 if the codes are every item exactly once.
+    if (argc > 1 && !strcmp(argv[1], "items")) {
+        std::vector<int32_t> per, codes;
+        int i = 2;
+        for (; i < argc && strcmp(argv[i], "--"); i++) per.push_back(atoi(argv[i]));
+        for (i++; i < argc; i++) codes.push_back(atoi(argv[i]));
+        printf("%d\n", vx::is_item_schedule(codes.data(), (int32_t)codes.size(), per.data(),
+                                            (int32_t)per.size()) ? 1 : 0);
+        return 0;
+    }
     const int n = argc - 1;
     int32_t *a = n ? (int32_t *)malloc(sizeof(int32_t) * n) : nullptr;
     for (int i = 0; i < n; i++) a[i] = (int32_t)atoi(argv[i + 1]);
@@ -58,3 +72,24 @@ def test_corruptions_are_refused(checker):
     assert not checker([0, 0, 1])        # duplicate (and one index missing)
     assert not checker([0, -1, 1])       # negative
     assert not checker([2, 1, 3])        # off by one
+
+
+def test_item_lists_are_accepted(checker):
+    # chunks 0 and 2 are 64^3 (four slab items each), chunk 1 one item
+    per = [4, 1, 4]
+    items = [0 << 2 | q for q in range(4)] + [1 << 2] + [2 << 2 | q for q in range(4)]
+    assert checker(["items", *per, "--", *items])
+    assert checker(["items", *per, "--", *reversed(items)])
+    assert checker(["items", "--"])
+
+
+def test_item_corruptions_are_refused(checker):
+    per = [4, 1, 4]
+    good = [0, 1, 2, 3, 4, 8, 9, 10, 11]
+    assert not checker(["items", *per, "--", *good[:-1]])           # an item missing
+    assert not checker(["items", *per, "--", *good[:-1], 8])        # a duplicate
+    assert not checker(["items", *per, "--", *good[:-1], 5])        # slab 1 of a one-item chunk
+    assert not checker(["items", *per, "--", *good[:-1], 12])       # chunk out of range
+    assert not checker(["items", *per, "--", *good[:-1], -1])       # negative
+    assert not checker(["items", 5, "--", 0, 1, 2, 3, 4])          # five items per chunk
+    assert not checker(["items", *per, "--", *good, 11])            # one code too many

@@ -9,7 +9,9 @@
 //   store   per chunk i at chunk_off[i]: ids (N^3 B) | occupancy rows
 //           (N^2 * W u32) | 4 boundary planes (4 * N * W u32) | meta (solid block id, u32),
 //           each 256-byte aligned.
-//   ctl     counts[n] u32 | bases[n] u64 | total u64 | overflow u32
+//   ctl     counts[n] u32 | bases[n] u64 | total u64 | overflow u32 |
+//           emit_order[n_items] i32 | slabs[4 n] u32 (batches holding 64^3 chunks,
+//           whose emit works on y-slabs: n_items = n + 3 * (number of 64^3 chunks))
 //   arena   instances (68 B/face) | vertices (80 B/face) | indices (24 B/face)
 //           for `capacity` faces; chunk i's faces are [bases[i], +counts[i]).
 #include "vx_host.h"
@@ -129,17 +131,18 @@ bool check_schedules() {
 }
 
 bool schedules_ok(vx_batch *b) {
-    std::vector<int32_t> h((size_t)b->n);
-    const int32_t *arrays[2] = {b->emit_order, b->order_dev};
-    for (const int32_t *a : arrays) {
-        if (!a) continue;
-        if (hipMemcpy(h.data(), a, (size_t)b->n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
-            (void)hipGetLastError();
-            return false;
-        }
-        if (!is_schedule(h.data(), b->n)) return false;
-    }
-    return true;
+    std::vector<int32_t> h((size_t)std::max(b->n, b->n_items));
+    auto fetch = [&](const int32_t *a, int32_t m) {
+        if (hipMemcpy(h.data(), a, (size_t)m * 4, hipMemcpyDeviceToHost) == hipSuccess) return true;
+        (void)hipGetLastError();
+        return false;
+    };
+    if (b->order_dev && !(fetch(b->order_dev, b->n) && is_schedule(h.data(), b->n))) return false;
+    if (!fetch(b->emit_order, b->n_items)) return false;
+    if (!b->slabs) return is_schedule(h.data(), b->n);
+    std::vector<int32_t> per(b->n);
+    for (int i = 0; i < b->n; i++) per[i] = b->reqs[i].edge == 64 ? kSlabs : 1;
+    return is_item_schedule(h.data(), b->n_items, per.data(), b->n);
 }
 
 hipError_t enqueue_fill(vx_batch *b) {
@@ -148,15 +151,16 @@ hipError_t enqueue_fill(vx_batch *b) {
                        ctx->lut_dev, b->overflow + 1, ctx->stream);
 }
 hipError_t enqueue_count(vx_batch *b) {
-    return launch_count(b->mesh_dev, b->n, b->max_edge, b->counts, b->ctx->stream);
+    return launch_count(b->mesh_dev, b->n, b->max_edge, b->counts, b->slabs, b->ctx->stream);
 }
 hipError_t enqueue_scan(vx_batch *b) {
-    return launch_scan(b->counts, b->n, b->bases, b->total, b->overflow, b->emit_order,
-                       b->ctx->stream);
+    return launch_scan(b->counts, b->n, b->slabs ? b->mesh_dev : nullptr, b->bases, b->total,
+                       b->overflow, b->emit_order, b->ctx->stream);
 }
 hipError_t enqueue_emit(vx_batch *b) {
-    return launch_emit(b->mesh_dev, b->emit_order, b->n, b->max_edge, b->bases, b->counts, b->out,
-                       b->ctx->tables_dev, b->overflow, b->ctx->stream);
+    return launch_emit(b->mesh_dev, b->emit_order, b->n, b->n_items, b->max_edge, b->bases,
+                       b->counts, b->slabs, b->out, b->ctx->tables_dev, b->overflow,
+                       b->ctx->stream);
 }
 
 // The form a run takes.  VX_BATCH_AUTO runs the staged form unless the
@@ -370,13 +374,23 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
     }
     if ((st = ctx->dev_alloc((size_t)n * sizeof(FillWork), (void **)&b->fill_dev))) return fail(st);
     if ((st = ctx->dev_alloc((size_t)n * sizeof(MeshWork), (void **)&b->mesh_dev))) return fail(st);
-    const size_t ctl_bytes = up256((size_t)n * 4) + up256((size_t)n * 8) + 256 + up256((size_t)n * 4);
+    const bool slabbed = b->max_edge > 32;
+    b->n_items = n;
+    if (slabbed)
+        for (int i = 0; i < n; i++) b->n_items += reqs[i].edge == 64 ? kSlabs - 1 : 0;
+    const size_t ctl_bytes = up256((size_t)n * 4) + up256((size_t)n * 8) + 256 +
+                             up256((size_t)b->n_items * 4) +
+                             (slabbed ? up256((size_t)kSlabs * n * 4) : 0);
     if ((st = ctx->dev_alloc(ctl_bytes, &b->ctl))) return fail(st);
     b->counts = (uint32_t *)b->ctl;
     b->bases = (uint64_t *)((char *)b->ctl + up256((size_t)n * 4));
     b->total = (uint64_t *)((char *)b->bases + up256((size_t)n * 8));
     b->overflow = (uint32_t *)(b->total + 1);  // [0] arena overflow, [1] schedule error
     b->emit_order = (int32_t *)((char *)b->total + 256);
+    if (slabbed) {
+        b->slabs = (uint32_t *)((char *)b->emit_order + up256((size_t)b->n_items * 4));
+        VX_HIP(ctx, hipMemsetAsync(b->slabs, 0, (size_t)kSlabs * n * 4, ctx->stream));
+    }
     VX_HIP(ctx, hipMemsetAsync(b->overflow, 0, 8, ctx->stream));
 
     std::vector<FillWork> fw(n);
@@ -472,9 +486,16 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
     }
     {
         // the scan rewrites it every run; start from a valid order regardless
-        std::vector<int32_t> iota_order(n);
-        std::iota(iota_order.begin(), iota_order.end(), 0);
-        VX_HIP(ctx, hipMemcpyAsync(b->emit_order, iota_order.data(), (size_t)n * 4,
+        // (chunk indices, or every emit item's code chunk << 2 | slab)
+        std::vector<int32_t> iota_order;
+        iota_order.reserve(b->n_items);
+        for (int i = 0; i < n; i++) {
+            if (!b->slabs) iota_order.push_back(i);
+            else
+                for (int q = 0; q < (reqs[i].edge == 64 ? kSlabs : 1); q++)
+                    iota_order.push_back(i << 2 | q);
+        }
+        VX_HIP(ctx, hipMemcpyAsync(b->emit_order, iota_order.data(), (size_t)b->n_items * 4,
                                    hipMemcpyHostToDevice, ctx->stream));
         VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // pageable source
     }
@@ -575,12 +596,15 @@ extern "C" int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *b) {
     for (int i = 0; i < n; i++)
         VX_HIP(ctx, launch_fill(b->fill_dev + i, nullptr, 1, b->reqs[i].edge, ctx->terrains_dev,
                                 ctx->lut_dev, b->overflow + 1, ctx->stream));
+    auto slabs_of = [&](int i) { return b->slabs ? b->slabs + (size_t)kSlabs * i : nullptr; };
     for (int i = 0; i < n; i++)
-        VX_HIP(ctx, launch_count(b->mesh_dev + i, 1, b->reqs[i].edge, b->counts + i, ctx->stream));
+        VX_HIP(ctx, launch_count(b->mesh_dev + i, 1, b->reqs[i].edge, b->counts + i, slabs_of(i),
+                                 ctx->stream));
     VX_HIP(ctx, enqueue_scan(b));
     for (int i = 0; i < n; i++)
-        VX_HIP(ctx, launch_emit(b->mesh_dev + i, nullptr, 1, b->reqs[i].edge, b->bases + i,
-                                b->counts + i, b->out, ctx->tables_dev, b->overflow, ctx->stream));
+        VX_HIP(ctx, launch_emit(b->mesh_dev + i, nullptr, 1, 0, b->reqs[i].edge, b->bases + i,
+                                b->counts + i, slabs_of(i), b->out, ctx->tables_dev, b->overflow,
+                                ctx->stream));
     b->ran = true;
     b->results = false;
     return VX_OK;
@@ -628,6 +652,8 @@ extern "C" int vx_batch_wait(vx_ctx *ctx, vx_batch *b, uint64_t *total_faces) {
             b->results = false;
             return st;
         }
+        // a persistent form writes no slab counts: count again first
+        if (b->slabs) VX_HIP(ctx, enqueue_count(b));
         VX_HIP(ctx, enqueue_scan(b));
         VX_HIP(ctx, enqueue_emit(b));
         VX_HIP(ctx, hipMemcpyAsync(ctx->pinned + 8, b->overflow, 8, hipMemcpyDeviceToHost,

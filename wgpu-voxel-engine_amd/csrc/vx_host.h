@@ -140,8 +140,11 @@ struct vx_batch {
     vx::MeshWork *mesh_dev = nullptr;
     int32_t *order_dev = nullptr;   // longest-first schedule (null when uniform)
     int32_t *coords_dev = nullptr;  // chunk coordinates (draw lists), made on first use
-    void *ctl = nullptr;            // counts[n] | bases[n] | total | overflow | emit_order[n]
+    void *ctl = nullptr;            // counts[n] | bases[n] | total | overflow | emit_order[n_items]
+                                    // | slabs[kSlabs * n] (batches holding 64^3 chunks)
     int32_t *emit_order = nullptr;  // emission order written by the scan (largest first)
+    int32_t n_items = 0;            // emit items: n, or with slabs kSlabs per 64^3 chunk
+    uint32_t *slabs = nullptr;      // per-slab face counts of 64^3 chunks (null when none)
     uint32_t *counts = nullptr;
     uint64_t *bases = nullptr;
     uint64_t *total = nullptr;

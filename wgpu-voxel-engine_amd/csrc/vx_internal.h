@@ -161,19 +161,26 @@ inline int words_per_row(int edge) { return (edge + 31) / 32; }
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const TerrainDev *terrains, const GradLut *lut, uint32_t *error,
                        hipStream_t s);
+// Batches that hold 64^3 chunks emit by items: each 64^3 chunk is kSlabs
+// y-slabs of kSlabLayers layers (face counts per slab in slabs[kSlabs * i
+// ..], written by the count kernel), any other chunk one item.  With `items`
+// (the batch's mesh work) the scan's order lists item codes chunk << 2 |
+// slab, at most kSlabs * n of them; launch_emit takes their number.
+constexpr int kSlabs = 4;
 hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint32_t *counts,
-                        hipStream_t s);
-hipError_t launch_scan(const uint32_t *counts, int32_t n, uint64_t *bases, uint64_t *total,
-                       uint32_t *overflow, int32_t *order_out, hipStream_t s);
+                        uint32_t *slabs, hipStream_t s);
+hipError_t launch_scan(const uint32_t *counts, int32_t n, const MeshWork *items, uint64_t *bases,
+                       uint64_t *total, uint32_t *overflow, int32_t *order_out, hipStream_t s);
 hipError_t launch_fill_one(const FillWork &w, const TerrainDev *terrains, const GradLut *lut,
                            hipStream_t s);
 hipError_t launch_pack_one(const FillWork &w, hipStream_t s);
 hipError_t launch_count_one(const MeshWork &w, uint32_t *count, hipStream_t s);
 hipError_t launch_emit_one(const MeshWork &w, uint64_t base, uint32_t faces, MeshOut out,
                            const MeshTables *tables, uint32_t *overflow, hipStream_t s);
-hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t max_edge,
-                       const uint64_t *bases, const uint32_t *counts, MeshOut out,
-                       const MeshTables *tables, uint32_t *overflow, hipStream_t s);
+hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t n_items,
+                       int32_t max_edge, const uint64_t *bases, const uint32_t *counts,
+                       const uint32_t *slabs, MeshOut out, const MeshTables *tables,
+                       uint32_t *overflow, hipStream_t s);
 hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
                         const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
                         FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,

@@ -17,6 +17,7 @@
 #include "vx_internal.h"
 
 #include <mutex>
+#include <type_traits>
 
 namespace vx {
 
@@ -138,34 +139,43 @@ __device__ __forceinline__ void line_column(int li, int lane, const Halo &H, int
     else { wx = ox + p; wz = oz + N; }
 }
 
-// Stores the bits of one ballot taken at height y by line group g.
-template <int N, int MAXN, class SH>
-__device__ __forceinline__ void store_lines(SH &S, const Halo &H, uint64_t b, int y, int g,
-                                            int lane) {
-    constexpr int RS = RowStride<MAXN>::v;
-    uint32_t *occ = reinterpret_cast<uint32_t *>(S.occ);
-    if (N == 64) {
-        if (lane < 2) {
-            const uint32_t v = lane ? (uint32_t)(b >> 32) : (uint32_t)b;
-            const int li = g;
-            if (li < N) occ[(y * N + li) * 2 + lane] = v;
-            else if (li - N < H.n) reinterpret_cast<uint32_t *>(S.nb[H.side(li - N)])[y * 2 + lane] = v;
+// Rows of one line group, one per lane, gathered from the per-height
+// ballots: lane k keeps the row at height k % N of line g * L + k / N (the
+// ballot's bits [k / N * N, +N)).  Keeping costs a compare and a select per
+// height; the rows then go to LDS in one store per lane (put_rows) instead of
+// one two-lane store per height.
+template <int N> struct RowKeep {
+    uint32_t w0 = 0u, w1 = 0u;  // the row (w1: bits 32..63 at N = 64)
+};
+
+template <int N>
+__device__ __forceinline__ void keep_row(RowKeep<N> &K, uint64_t b, int y, int lane) {
+    if constexpr (N == 64) {
+        if (lane == y) {
+            K.w0 = (uint32_t)b;
+            K.w1 = (uint32_t)(b >> 32);
         }
     } else {
-        constexpr int L = 64 / N;
         constexpr uint32_t MASK = N >= 32 ? 0xFFFFFFFFu : ((1u << (N & 31)) - 1u);
-        if (lane < L) {
-            const int li = g * L + lane;
-            const uint32_t v = (uint32_t)(b >> (lane * N)) & MASK;
-            uint32_t *dst = nullptr;
-            if (li < N) dst = occ + (y * N + li) * RS;
-            else if (li - N < H.n) dst = reinterpret_cast<uint32_t *>(S.nb[H.side(li - N)]) + y * RS;
-            if (dst) {
-                dst[0] = v;
-                if (RS == 2) dst[1] = 0u;
-            }
-        }
+        if ((lane & (N - 1)) == y) K.w0 = (uint32_t)(b >> (lane & ~(N - 1))) & MASK;
     }
+}
+
+// Stores the kept rows of heights [y_lo, y_hi) of line group g.
+template <int N, int MAXN, class SH>
+__device__ __forceinline__ void put_rows(SH &S, const Halo &H, const RowKeep<N> &K, int g,
+                                         int lane, int y_lo, int y_hi) {
+    constexpr int RS = RowStride<MAXN>::v;
+    constexpr int L = 64 / N;
+    const int y = lane & (N - 1);
+    if (y < y_lo || y >= y_hi) return;
+    const int li = g * L + lane / N;
+    uint32_t *dst = nullptr;
+    if (li < N) dst = reinterpret_cast<uint32_t *>(S.occ) + (y * N + li) * RS;
+    else if (li - N < H.n) dst = reinterpret_cast<uint32_t *>(S.nb[H.side(li - N)]) + y * RS;
+    if (!dst) return;
+    dst[0] = K.w0;
+    if (RS == 2) dst[1] = N == 64 ? K.w1 : 0u;
 }
 
 // The 8 corner hashes of a lattice cell for the column-hoisted evaluation,
@@ -290,14 +300,16 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
                 f = f * T.lacunarity;
                 amp = amp * T.gain;
             }
+            RowKeep<N> K;
 #pragma unroll 1
             for (int yy = 0; yy < SEG; yy++) {
                 const int y = y0 + yy;
                 const bool solid = solid_of<REF>(sum[yy], T, oy + y);
-                store_lines<N, MAXN>(S, H, __ballot(solid), y, g, lane);
+                keep_row<N>(K, __ballot(solid), y, lane);
                 if (DBG && li < N)
                     noise_out[(size_t)(wxi - ox) + (size_t)N * li + (size_t)N * N * y] = sum[yy];
             }
+            put_rows<N, MAXN>(S, H, K, g, lane, y0, y0 + SEG);
         }
     }
 }
@@ -326,7 +338,9 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
         float f = T.frequency, a = 1.0f;
 #pragma unroll
         for (int o = 0; o < OCT; o++) {
-            fo[o] = f;
+            // uniform: the octave frequencies (used per item, not per row) go
+            // to SGPRs, which leaves their VGPRs to the row loop
+            fo[o] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(f)));
             ao[o] = a;
             f = f * T.lacunarity;
             a = a * T.gain;
@@ -358,6 +372,7 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
             wz[o] = fade(Z - fzl);
             Zi[o] = cell8(fzl);
         }
+        RowKeep<N> K;
 #pragma unroll 1
         for (int y = y0; y < y0 + SEG; y++) {
             uint32_t chg = y == y0 ? (1u << OCT) - 1u
@@ -416,10 +431,11 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
                 const float n = blend8<REF>(u[o], wz[o], t.z, g0, g1, g2, g3, g4, g5, g6, g7);
                 sum = sum + ao[o] * n;
             }
-            store_lines<N, MAXN>(S, H, __ballot(solid_of<REF>(sum, T, oy + y)), y, g, lane);
+            keep_row<N>(K, __ballot(solid_of<REF>(sum, T, oy + y)), y, lane);
             if (DBG && li < N)
                 noise_out[(size_t)(wxi - ox) + (size_t)N * li + (size_t)N * N * y] = sum;
         }
+        put_rows<N, MAXN>(S, H, K, g, lane, y0, y0 + SEG);
     }
 }
 
@@ -448,7 +464,9 @@ __device__ void heightmap_lines(const TerrainDev &T, int ox, int oy, int oz, con
         else if (hf > (float)T.height_max) h = T.height_max;
         else h = (int)hf;
         if (DBG && li < N) noise_out[(size_t)(wxi - ox) + (size_t)N * li] = sum;
-        for (int y = 0; y < N; y++) store_lines<N, MAXN>(S, H, __ballot(oy + y <= h), y, g, lane);
+        RowKeep<N> K;
+        for (int y = 0; y < N; y++) keep_row<N>(K, __ballot(oy + y <= h), y, lane);
+        put_rows<N, MAXN>(S, H, K, g, lane, 0, N);
     }
 }
 
@@ -783,6 +801,21 @@ __global__ void __launch_bounds__(kBlock) pack_one_kernel(FillWork w) {
 // HBM write rate of contiguous 16-B lanes.
 constexpr int kStageWords = 32 * 20;  // 32 records per wave pass: 2.5 KB
 
+// Rows of a descriptor tile with more than kHeavyFaces faces, walked by a
+// whole wave (one lane per voxel of the row) instead of one lane: the rows
+// at a chunk face whose neighbour is absent, or the bottom layer, hold a
+// face at nearly every voxel, and a lane walking 64 of them serially holds
+// its wave (and the workgroup's next barrier) for 64 steps while its other
+// lanes walk rows of two or three faces.  At most tile / (kHeavyFaces + 1)
+// + 2 rows of a tile are heavy; a row beyond the list's room is walked by
+// its lane.
+constexpr int kHeavyFaces = 24;
+constexpr int kHeavyMax = kDescTile / 8;
+struct HeavyRows {
+    uint16_t row[kHeavyMax];
+    uint32_t n;
+};
+
 // BLOCK threads per workgroup: kBlock, or 512 for the MAXN = 64 emit, whose
 // 32 KB of u64 rows and 16 KB of row offsets allow only 2 workgroups per CU
 // (8 waves of 512 threads keep 4 waves per SIMD storing instead of 2).
@@ -797,8 +830,35 @@ template <int MAXN, int BLOCK = kBlock> struct MeshShared {
     float Mb[6][2][3][3];
     uint32_t tex[3][6];
     uint32_t red[BLOCK / 64];
+    HeavyRows heavy;
     float4 stage[BLOCK / 64][kStageWords / 4];
 };
+
+// LDS of one y-slab of a 64^3 chunk (the emit of batches that hold 64^3
+// chunks works on slabs of kSlabLayers layers, whose face counts the count
+// kernel writes): occupancy rows of layers ylo - 1 .. ylo + kSlabLayers
+// (zero outside the chunk), the four neighbour planes' rows of the slab's
+// layers, and the emit's descriptor tile, row firsts and staging.  33 KB, so
+// four 256-thread workgroups per CU instead of the two that a whole 64^3
+// chunk's 32 KB of rows and 16 KB of row firsts allow.
+constexpr int kSlabLayers = 16;
+static_assert(kSlabs * kSlabLayers == 64, "slabs tile a 64^3 chunk");
+struct SlabShared {
+    uint64_t occ[(kSlabLayers + 2) * 64];
+    uint64_t nb[4][kSlabLayers];
+    uint32_t desc[kDescTile / 2];
+    uint32_t rowfirst[kSlabLayers * 64];
+    float P[6][2][4][3];
+    float Mb[6][2][3][3];
+    uint32_t tex[3][6];
+    uint32_t red[kBlock / 64];
+    HeavyRows heavy;
+    float4 stage[kBlock / 64][kStageWords / 4];
+};
+
+// Rows an emit workgroup scans: the whole chunk, or one slab.
+template <class SH, int N> struct RowsOf { static constexpr int v = N * N; };
+template <int N> struct RowsOf<SlabShared, N> { static constexpr int v = kSlabLayers * N; };
 
 // Orders the LDS accesses of one wave: staging written by some lanes is read
 // by others of the same wave (LDS executes one wave's operations in order).
@@ -829,6 +889,23 @@ __device__ __forceinline__ void load_rows(const MeshWork &w, SH &S) {
     }
 }
 
+// face_masks for local layer yl of a slab held in SlabShared (its halo
+// layers hold the rows above and below, zero outside the chunk).
+__device__ __forceinline__ void slab_face_masks(const SlabShared &S, int yl, int z, uint64_t m[6]) {
+    const int b = (yl + 1) * 64 + z;
+    const uint64_t self = S.occ[b];
+    const uint64_t lb = (S.nb[0][yl] >> z) & 1ull;
+    const uint64_t rb = (S.nb[1][yl] >> z) & 1ull;
+    m[0] = self & ~((self << 1) | lb);
+    m[1] = self & ~((self >> 1) | (rb << 63));
+    const uint64_t back = z > 0 ? S.occ[b - 1] : S.nb[2][yl];
+    const uint64_t front = z < 63 ? S.occ[b + 1] : S.nb[3][yl];
+    m[2] = self & ~back;
+    m[3] = self & ~front;
+    m[4] = self & ~S.occ[b - 64];
+    m[5] = self & ~S.occ[b + 64];
+}
+
 // Six face masks of row (y, z), bit x set iff voxel x emits that face
 // (chunk.rs:77-240): neighbour cell air, or outside the chunk with the
 // neighbour chunk absent, or outside vertically.
@@ -836,19 +913,39 @@ template <int N, int MAXN, class SH>
 __device__ __forceinline__ void face_masks(const SH &S, int y, int z,
                                            typename RowType<MAXN>::T m[6]) {
     using R = typename RowType<MAXN>::T;
-    const R self = S.occ[y * N + z];
-    const R lb = (S.nb[0][y] >> z) & (R)1;
-    const R rb = (S.nb[1][y] >> z) & (R)1;
-    m[0] = self & ~((self << 1) | lb);
-    m[1] = self & ~((self >> 1) | (rb << (N - 1)));
-    const R back = z > 0 ? S.occ[y * N + z - 1] : S.nb[2][y];
-    const R front = z < N - 1 ? S.occ[y * N + z + 1] : S.nb[3][y];
-    m[2] = self & ~back;
-    m[3] = self & ~front;
-    const R down = y > 0 ? S.occ[(y - 1) * N + z] : (R)0;
-    const R up = y < N - 1 ? S.occ[(y + 1) * N + z] : (R)0;
-    m[4] = self & ~down;
-    m[5] = self & ~up;
+    if constexpr (std::is_same<SH, SlabShared>::value) {
+        static_assert(N == 64 && MAXN == 64, "slabs are layers of 64^3 chunks");
+        slab_face_masks(S, y, z, m);
+    } else {
+        const R self = S.occ[y * N + z];
+        const R lb = (S.nb[0][y] >> z) & (R)1;
+        const R rb = (S.nb[1][y] >> z) & (R)1;
+        m[0] = self & ~((self << 1) | lb);
+        m[1] = self & ~((self >> 1) | (rb << (N - 1)));
+        const R back = z > 0 ? S.occ[y * N + z - 1] : S.nb[2][y];
+        const R front = z < N - 1 ? S.occ[y * N + z + 1] : S.nb[3][y];
+        m[2] = self & ~back;
+        m[3] = self & ~front;
+        const R down = y > 0 ? S.occ[(y - 1) * N + z] : (R)0;
+        const R up = y < N - 1 ? S.occ[(y + 1) * N + z] : (R)0;
+        m[4] = self & ~down;
+        m[5] = self & ~up;
+    }
+}
+
+// Loads slab `ylo / kSlabLayers` of a 64^3 chunk (rows are u32 pairs, low
+// word first) with its halo layers and neighbour-plane rows.
+__device__ __forceinline__ void load_slab(const MeshWork &w, SlabShared &S, int ylo) {
+    const uint64_t *occ = reinterpret_cast<const uint64_t *>(w.occ);
+    for (int i = threadIdx.x; i < (kSlabLayers + 2) * 64; i += kBlock) {
+        const int y = ylo - 1 + (i >> 6);
+        S.occ[i] = (y >= 0 && y < 64) ? occ[y * 64 + (i & 63)] : 0ull;
+    }
+    for (int i = threadIdx.x; i < 4 * kSlabLayers; i += kBlock) {
+        const int k = i / kSlabLayers, yl = i - k * kSlabLayers;
+        const uint64_t *p = reinterpret_cast<const uint64_t *>(w.nb[k]);
+        S.nb[k][yl] = p ? p[ylo + yl] : 0ull;
+    }
 }
 
 // Visible faces of the chunk whose rows and neighbour planes are in LDS.
@@ -872,35 +969,76 @@ template <int MAXN> struct CountShared {
     uint32_t red[kBlock / 64];
 };
 
+// A 64^3 chunk's count also goes out per slab (slabs[0..kSlabs), when
+// given): thread t's rows t + kBlock * i lie in slab i / (kSlabLayers * 64 /
+// kBlock).
 template <int N, int MAXN>
-__device__ void count_chunk(const MeshWork &w, CountShared<MAXN> &S, uint32_t *count_out) {
+__device__ void count_chunk(const MeshWork &w, CountShared<MAXN> &S, uint32_t *count_out,
+                            uint32_t *slabs) {
     load_rows<N, MAXN>(w, S);
     __syncthreads();
-    const uint32_t total = count_rows<N, MAXN>(S);
-    if (threadIdx.x == 0) *count_out = total;
+    if constexpr (N == 64) {
+        using R = typename RowType<MAXN>::T;
+        constexpr int PER = kSlabLayers * 64 / kBlock;  // a thread's rows per slab
+        // per-slab sums: wave reductions, then one barrier
+        __shared__ uint32_t s_part[kBlock / 64][kSlabs];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll 1
+        for (int s = 0; s < kSlabs; s++) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int r = threadIdx.x + kBlock * (s * PER + i);
+                R m[6];
+                face_masks<N, MAXN>(S, r / N, r % N, m);
+#pragma unroll
+                for (int d = 0; d < 6; d++) c += popc(m[d]);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+            if (lane == 0) s_part[wave][s] = c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (int s = 0; s < kSlabs; s++) {
+                uint32_t c = 0;
+                for (int k = 0; k < kBlock / 64; k++) c += s_part[k][s];
+                if (slabs) slabs[s] = c;
+                total += c;
+            }
+            *count_out = total;
+        }
+    } else {
+        const uint32_t total = count_rows<N, MAXN>(S);
+        if (threadIdx.x == 0) *count_out = total;
+    }
 }
 
 template <int MAXN>
-__device__ __forceinline__ void count_item(const MeshWork &w, CountShared<MAXN> &S, uint32_t *out) {
+__device__ __forceinline__ void count_item(const MeshWork &w, CountShared<MAXN> &S, uint32_t *out,
+                                           uint32_t *slabs) {
     switch (w.edge) {
-    case 16: count_chunk<16, MAXN>(w, S, out); break;
-    case 32: if (MAXN >= 32) count_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, S, out); break;
-    case 64: if (MAXN >= 64) count_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(w, S, out); break;
+    case 16: count_chunk<16, MAXN>(w, S, out, slabs); break;
+    case 32: if (MAXN >= 32) count_chunk<(MAXN >= 32 ? 32 : 16), MAXN>(w, S, out, slabs); break;
+    case 64: if (MAXN >= 64) count_chunk<(MAXN >= 64 ? 64 : 16), MAXN>(w, S, out, slabs); break;
     default: break;
     }
 }
 
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) count_kernel(const MeshWork *__restrict__ work,
-                                                       uint32_t *__restrict__ counts) {
+                                                       uint32_t *__restrict__ counts,
+                                                       uint32_t *__restrict__ slabs) {
     __shared__ CountShared<MAXN> S;
-    count_item<MAXN>(work[blockIdx.x], S, counts + blockIdx.x);
+    count_item<MAXN>(work[blockIdx.x], S, counts + blockIdx.x,
+                     slabs ? slabs + (size_t)kSlabs * blockIdx.x : nullptr);
 }
 
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) count_one_kernel(MeshWork w, uint32_t *__restrict__ count) {
     __shared__ CountShared<MAXN> S;
-    count_item<MAXN>(w, S, count);
+    count_item<MAXN>(w, S, count, nullptr);
 }
 
 // Emission order: chunks by descending face count (buckets of 16 faces,
@@ -914,8 +1052,11 @@ __device__ __forceinline__ int order_bucket(uint32_t c) {
 
 // Device-wide exclusive scan of per-chunk face counts (one workgroup of
 // 1024 threads, each owning a contiguous run of chunks); with order_out it
-// also writes the emission order (a counting sort by face count).
+// also writes the emission order (a counting sort by face count).  With
+// `items` (the batch's mesh work) the order lists emit items as codes
+// chunk << 2 | slab: kSlabs per 64^3 chunk, one (slab 0) per other chunk.
 __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__ counts, int32_t n,
+                                                    const MeshWork *__restrict__ items,
                                                     uint64_t *__restrict__ bases,
                                                     uint64_t *__restrict__ total,
                                                     uint32_t *__restrict__ overflow,
@@ -931,6 +1072,13 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     uint32_t cv[kCached];
 #pragma unroll
     for (int j = 0; j < kCached; j++) cv[j] = b + j < e ? counts[b + j] : 0u;
+    auto items_of = [&](int i) -> uint32_t {
+        return items && items[i].edge == 64 ? (uint32_t)kSlabs : 1u;
+    };
+    // emit items per cached chunk, loaded with the counts
+    uint32_t iv[kCached];
+#pragma unroll
+    for (int j = 0; j < kCached; j++) iv[j] = b + j < e && order_out ? items_of(b + j) : 1u;
     const int ec = min(e, b + kCached);
     uint64_t local = 0;
 #pragma unroll
@@ -969,8 +1117,8 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kCached; j++)
-        if (b + j < ec) atomicAdd(&s_hist[order_bucket(cv[j])], 1u);
-    for (int i = b + kCached; i < e; i++) atomicAdd(&s_hist[order_bucket(counts[i])], 1u);
+        if (b + j < ec) atomicAdd(&s_hist[order_bucket(cv[j])], iv[j]);
+    for (int i = b + kCached; i < e; i++) atomicAdd(&s_hist[order_bucket(counts[i])], items_of(i));
     __syncthreads();
     const uint32_t cnt = s_hist[tid];
     uint32_t cincl = cnt;
@@ -986,11 +1134,18 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
     for (int i = 0; i < wave; i++) cbase += s_w32[i];
     s_hist[tid] = cbase + cincl - cnt;
     __syncthreads();
+    auto place = [&](int i, uint32_t c, uint32_t k) {
+        const uint32_t pos = atomicAdd(&s_hist[order_bucket(c)], k);
+        if (!items) {
+            order_out[pos] = i;
+            return;
+        }
+        for (uint32_t q = 0; q < k; q++) order_out[pos + q] = (i << 2) | (int)q;
+    };
 #pragma unroll
     for (int j = 0; j < kCached; j++)
-        if (b + j < ec) order_out[atomicAdd(&s_hist[order_bucket(cv[j])], 1u)] = b + j;
-    for (int i = b + kCached; i < e; i++)
-        order_out[atomicAdd(&s_hist[order_bucket(counts[i])], 1u)] = i;
+        if (b + j < ec) place(b + j, cv[j], iv[j]);
+    for (int i = b + kCached; i < e; i++) place(i, counts[i], items_of(i));
 }
 
 // Descriptors (row | x << 12 | dir << 18) of row r's faces that fall in the
@@ -1017,6 +1172,29 @@ __device__ __forceinline__ void walk_row(SH &S, int r, uint32_t f, uint32_t t0, 
     }
 }
 
+// The faces of row r at voxel x (one lane of a wave walking the row): the
+// row's faces before x (in (x, direction) order) come from the masks below
+// x, then x's own faces follow in direction order.
+template <int N, int MAXN, class SH>
+__device__ __forceinline__ void walk_row_at(SH &S, int r, int x, uint32_t f, uint32_t t0,
+                                            uint32_t t1) {
+    using R = typename RowType<MAXN>::T;
+    R m[6];
+    face_masks<N, MAXN>(S, r / N, r % N, m);
+    const R below = ((R)1 << x) - (R)1;
+    uint32_t g = f;
+#pragma unroll
+    for (int d = 0; d < 6; d++) g += popc(m[d] & below);
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        if ((m[d] >> x) & 1) {
+            if (g >= t0 && g < t1)
+                S.desc[g - t0] = (uint32_t)r | ((uint32_t)x << 12) | ((uint32_t)d << 18);
+            g++;
+        }
+    }
+}
+
 // Emits one chunk's faces.  Each thread owns RPT consecutive rows; a block
 // scan of the per-thread counts gives every row its first face index, so the
 // face order is (y, z, x, direction) exactly as chunk.rs:65-240 pushes them.
@@ -1030,14 +1208,18 @@ __device__ __forceinline__ void walk_row(SH &S, int r, uint32_t f, uint32_t t0, 
 // ownership (4 or 16 rows per thread) would leave to a handful of threads.
 // `parts` workgroups may share a chunk (small batches): part p emits tiles
 // p, p + parts, ... and the matching share of the index stores; each part
-// repeats the row scan.
-template <int N, int MAXN, bool INTERLEAVE, int BLOCK = kBlock, class SH>
+// repeats the row scan.  With SlabShared the rows are those of one slab,
+// layers ylo .. ylo + kSlabLayers - 1, whose first face is face `off` of
+// the chunk (indices are chunk-local).
+template <int N, int MAXN, bool INTERLEAVE, int BLOCK = kBlock, bool HEAVY = INTERLEAVE,
+          class SH>
 __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const MeshOut &out,
-                          SH &S, int part = 0, int parts = 1) {
+                          SH &S, int part = 0, int parts = 1, int ylo = 0, uint32_t off = 0) {
     using R = typename RowType<MAXN>::T;
+    constexpr int ROWS = RowsOf<SH, N>::v;
     // rows per thread (a 16^3 chunk in a 512-thread workgroup: half the
     // threads own one row, the others none)
-    constexpr int RPT = (N * N + BLOCK - 1) / BLOCK;
+    constexpr int RPT = (ROWS + BLOCK - 1) / BLOCK;
     const int tid = threadIdx.x;
     const bool want_v = (out.formats & kFmtVertices) != 0;
     const bool want_i = (out.formats & kFmtInstances) != 0;
@@ -1048,7 +1230,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         uint2 *dst = reinterpret_cast<uint2 *>(out.indices + base * 6);
         for (uint32_t q = (uint32_t)(part * BLOCK + tid); q < 3 * F; q += (uint32_t)(parts * BLOCK)) {
             const uint32_t face = q / 3, part = q - face * 3;
-            const uint32_t b4 = 4 * face;
+            const uint32_t b4 = 4 * (face + off);
             uint2 v;
             if (part == 0) v = make_uint2(b4 + 0, b4 + 2);
             else if (part == 1) v = make_uint2(b4 + 1, b4 + 3);
@@ -1064,7 +1246,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         R m[6];
         const int r = r0 + i;
         uint32_t c = 0;
-        if (N * N % BLOCK == 0 || r < N * N) {
+        if (ROWS % BLOCK == 0 || r < ROWS) {
             face_masks<N, MAXN>(S, r / N, r % N, m);
 #pragma unroll
             for (int d = 0; d < 6; d++) c += popc(m[d]);
@@ -1079,21 +1261,47 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
         uint32_t f = first;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
-            if (N * N % BLOCK == 0 || r0 + i < N * N) S.rowfirst[r0 + i] = f;
+            if (ROWS % BLOCK == 0 || r0 + i < ROWS) S.rowfirst[r0 + i] = f;
             f += rc[i];
         }
+        if constexpr (HEAVY)
+            if (tid == 0) S.heavy.n = 0u;
         __syncthreads();
     }
 
     constexpr uint32_t TILE = sizeof(S.desc) / sizeof(S.desc[0]);
     for (uint32_t t0 = (uint32_t)part * TILE; t0 < F; t0 += (uint32_t)parts * TILE) {
         const uint32_t t1 = min(F, t0 + TILE);
-        if constexpr (INTERLEAVE) {
+        if constexpr (INTERLEAVE && !HEAVY) {
 #pragma unroll 1
-            for (int r = tid; r < N * N; r += BLOCK) {
+            for (int r = tid; r < ROWS; r += BLOCK) {
                 const uint32_t f = S.rowfirst[r];
-                const uint32_t e = r + 1 < N * N ? S.rowfirst[r + 1] : F;
+                const uint32_t e = r + 1 < ROWS ? S.rowfirst[r + 1] : F;
                 if (e > t0 && f < t1) walk_row<N, MAXN>(S, r, f, t0, t1);
+            }
+        } else if constexpr (INTERLEAVE) {
+#pragma unroll 1
+            for (int r = tid; r < ROWS; r += BLOCK) {
+                const uint32_t f = S.rowfirst[r];
+                const uint32_t e = r + 1 < ROWS ? S.rowfirst[r + 1] : F;
+                if (e > t0 && f < t1) {
+                    uint32_t slot = kHeavyMax;
+                    if (e - f > (uint32_t)kHeavyFaces) slot = atomicAdd(&S.heavy.n, 1u);
+                    if (slot < (uint32_t)kHeavyMax) S.heavy.row[slot] = (uint16_t)r;
+                    else walk_row<N, MAXN>(S, r, f, t0, t1);
+                }
+            }
+            __syncthreads();
+            // heavy rows: 64 / N rows per wave pass, lane = voxel
+            {
+                constexpr int RPP = 64 / N;
+                const int lane = tid & 63, wave = tid >> 6, sub = lane / N, x = lane - sub * N;
+                const uint32_t nh = min(S.heavy.n, (uint32_t)kHeavyMax);
+#pragma unroll 1
+                for (uint32_t h = (uint32_t)(wave * RPP + sub); h < nh; h += (uint32_t)(BLOCK / 64 * RPP)) {
+                    const int r = S.heavy.row[h];
+                    walk_row_at<N, MAXN>(S, r, x, S.rowfirst[r], t0, t1);
+                }
             }
         } else {
             uint32_t f = first;
@@ -1106,6 +1314,8 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             }
         }
         __syncthreads();
+        if constexpr (HEAVY)
+            if (tid == 0) S.heavy.n = 0u;  // read by every wave before the barrier above
         // Each wave takes 64 consecutive faces of the tile: lane i builds face
         // q0 + i in registers, stages it in the wave's LDS slot, then the wave
         // writes the 64 records as contiguous 16-B (vertices) or 4-B
@@ -1120,7 +1330,7 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             const uint32_t nv = min(64u, nt - q0);
             const uint32_t dsc = valid ? S.desc[q] : 0u;
             const int r = dsc & 0xFFF, x = (dsc >> 12) & 0x3F, d = (dsc >> 18) & 7;
-            const int y = r / N, z = r - y * N;
+            const int yl = r / N, z = r - yl * N, y = ylo + yl;
             const int wx = w.ox + x, wy = w.oy + y, wz = w.oz + z;
             const float t[3] = {(float)wx, (float)wy, (float)wz};
             const int sg[3] = {wx < 0, wy < 0, wz < 0};
@@ -1209,6 +1419,7 @@ __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint
         if (threadIdx.x == 0) atomicOr(overflow, 1u);
         return;
     }
+    if (F == 0) return;  // an air chunk (or one with no visible face)
     for (int i = threadIdx.x; i < 6 * 2 * 4 * 3; i += BLOCK) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
     for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += BLOCK) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = threadIdx.x; i < 18; i += BLOCK) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
@@ -1236,6 +1447,61 @@ emit_kernel(const MeshWork *__restrict__ work, const int32_t *__restrict__ order
     if (!order_entry_ok(item, n, overflow + 1)) return;
     emit_item<MAXN, BLOCK>(work[item], bases[item], counts[item], out, tables, overflow, S, part,
                            parts);
+}
+
+// One slab of a 64^3 chunk: its faces start `off` faces into the chunk,
+// off = the face counts of the slabs below it (sc = the chunk's slab counts).
+__device__ void emit_slab_item(const MeshWork &w, uint64_t chunk_base, const uint32_t *sc, int slab,
+                               const MeshOut &out, const MeshTables *tables, uint32_t *overflow,
+                               SlabShared &S, int part, int parts) {
+    uint32_t off = 0;
+    for (int q = 0; q < slab; q++) off += sc[q];
+    const uint32_t F = sc[slab];
+    const uint64_t base = chunk_base + off;
+    if (base + F > out.capacity) {
+        if (threadIdx.x == 0) atomicOr(overflow, 1u);
+        return;
+    }
+    if (F == 0) return;
+    for (int i = threadIdx.x; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
+    load_slab(w, S, slab * kSlabLayers);
+    __syncthreads();
+    emit_rows<64, 64, true, kBlock>(w, base, F, out, S, part, parts, slab * kSlabLayers, off);
+}
+
+// Emit of a batch that holds 64^3 chunks, over emit items (codes chunk << 2
+// | slab, the scan's order; with no order, code = workgroup / parts): a
+// slab of a 64^3 chunk, or a whole smaller chunk (slab 0; a code with
+// another slab for such a chunk sets the schedule error word).  Both kinds
+// share the workgroup's LDS, 33 KB, four workgroups per CU.
+union EmitShared64 {
+    MeshShared<32, kBlock> small;
+    SlabShared slab;
+};
+
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+emit_slab_kernel(const MeshWork *__restrict__ work, const int32_t *__restrict__ order,
+                 const uint64_t *__restrict__ bases, const uint32_t *__restrict__ counts,
+                 const uint32_t *__restrict__ slabs, MeshOut out,
+                 const MeshTables *__restrict__ tables, uint32_t *__restrict__ overflow, int parts,
+                 int32_t n) {
+    __shared__ EmitShared64 S;
+    const int k = (int)blockIdx.x / parts, part = (int)blockIdx.x - k * parts;
+    const int code = order ? order[k] : k;
+    const int item = code >> 2, slab = code & 3;
+    if (!order_entry_ok(item, n, overflow + 1)) return;
+    const MeshWork &w = work[item];
+    if (w.edge == 64) {
+        emit_slab_item(w, bases[item], slabs + (size_t)kSlabs * item, slab, out, tables, overflow,
+                       S.slab, part, parts);
+    } else if (slab == 0) {
+        emit_item<32, kBlock>(w, bases[item], counts[item], out, tables, overflow, S.small, part,
+                              parts);
+    } else if (order && threadIdx.x == 0) {
+        atomicOr(overflow + 1, 1u);
+    }
 }
 
 template <int MAXN>
@@ -1379,7 +1645,7 @@ __device__ void fused_chunk(const FusedWork &W, const TerrainDev &T, int t, int 
     if (base + F > out.capacity) {
         if (tid == 0) atomicOr(ctl.overflow, 1u);
     } else {
-        emit_rows<N, MAXN, (MAXN == 32)>(W.m, base, F, out, S);
+        emit_rows<N, MAXN, (MAXN == 32), kBlock, false>(W.m, base, F, out, S);
     }
     __syncthreads();
 }
@@ -1591,7 +1857,7 @@ __device__ void pipe_mesh(const FusedWork &W, int m, int n, const FusedCtl &ctl,
     if (base + F > out.capacity) {
         if (tid == 0) atomicOr(ctl.overflow, 1u);
     } else {
-        emit_rows<N, MAXN, true>(W.m, base, F, out, S);
+        emit_rows<N, MAXN, true, kBlock, false>(W.m, base, F, out, S);
     }
     __syncthreads();
 }
@@ -1784,18 +2050,18 @@ hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, in
 }
 
 hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint32_t *counts,
-                        hipStream_t s) {
+                        uint32_t *slabs, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (max_edge <= 16) count_kernel<16><<<n, kBlock, 0, s>>>(work, counts);
-    else if (max_edge <= 32) count_kernel<32><<<n, kBlock, 0, s>>>(work, counts);
-    else count_kernel<64><<<n, kBlock, 0, s>>>(work, counts);
+    if (max_edge <= 16) count_kernel<16><<<n, kBlock, 0, s>>>(work, counts, nullptr);
+    else if (max_edge <= 32) count_kernel<32><<<n, kBlock, 0, s>>>(work, counts, nullptr);
+    else count_kernel<64><<<n, kBlock, 0, s>>>(work, counts, slabs);
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const uint32_t *counts, int32_t n, uint64_t *bases, uint64_t *total,
-                       uint32_t *overflow, int32_t *order_out, hipStream_t s) {
+hipError_t launch_scan(const uint32_t *counts, int32_t n, const MeshWork *items, uint64_t *bases,
+                       uint64_t *total, uint32_t *overflow, int32_t *order_out, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    scan_kernel<<<1, 1024, 0, s>>>(counts, n, bases, total, overflow, order_out);
+    scan_kernel<<<1, 1024, 0, s>>>(counts, n, items, bases, total, overflow, order_out);
     return hipGetLastError();
 }
 
@@ -1829,17 +2095,20 @@ hipError_t launch_emit_one(const MeshWork &w, uint64_t base, uint32_t faces, Mes
     return hipGetLastError();
 }
 
-hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t max_edge,
-                       const uint64_t *bases, const uint32_t *counts, MeshOut out,
-                       const MeshTables *tables, uint32_t *overflow, hipStream_t s) {
+hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t n_items,
+                       int32_t max_edge, const uint64_t *bases, const uint32_t *counts,
+                       const uint32_t *slabs, MeshOut out, const MeshTables *tables,
+                       uint32_t *overflow, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    // small batches: up to 4 workgroups per chunk (8 when the batch holds
-    // 64^3 chunks, which reach 12 descriptor tiles of faces), so the grid
-    // still covers about a full GPU of workgroups (kEmitGrid of 256 threads,
-    // kEmitGrid64 of 512) and a chunk's tiles are emitted side by side
-    const int full = max_edge > 32 ? kEmitGrid64 : kEmitGrid;
-    const int parts = n >= full ? 1 : std::min(max_edge > 32 ? 8 : 4, (full + n - 1) / n);
-    const int grid = n * parts;
+    // items: chunks, or for batches that hold 64^3 chunks the scan's emit
+    // items (kSlabs codes per chunk without an order)
+    const int items = max_edge <= 32 ? n : order ? n_items : kSlabs * n;
+    if (items <= 0) return hipErrorInvalidValue;
+    // small batches: up to 4 workgroups per item, so the grid still covers
+    // about a full GPU of workgroups and an item's tiles are emitted side by
+    // side
+    const int parts = items >= kEmitGrid ? 1 : std::min(4, (kEmitGrid + items - 1) / items);
+    const int grid = items * parts;
     if (max_edge <= 16)
         emit_kernel<16, kBlock><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables,
                                                         overflow, parts, n);
@@ -1847,8 +2116,8 @@ hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, in
         emit_kernel<32, kBlock><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables,
                                                         overflow, parts, n);
     else
-        emit_kernel<64, kEmitBlock64><<<grid, kEmitBlock64, 0, s>>>(work, order, bases, counts, out,
-                                                                    tables, overflow, parts, n);
+        emit_slab_kernel<<<grid, kBlock, 0, s>>>(work, order, bases, counts, slabs, out, tables,
+                                                 overflow, parts, n);
     return hipGetLastError();
 }
 

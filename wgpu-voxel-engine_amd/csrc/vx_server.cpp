@@ -108,7 +108,9 @@ struct Flight {
     SharedBlock *arena = nullptr;     // mesh records
     FillWork *fill_dev = nullptr;
     MeshWork *mesh_dev = nullptr;
-    int32_t *order_dev = nullptr;
+    int32_t *order_dev = nullptr;     // fill schedule, then the scan's emission order
+    int32_t n_items = 0;              // emit items (kSlabs per 64^3 chunk when slabs is set)
+    uint32_t *slabs = nullptr;        // per-slab face counts (flights holding 64^3 chunks)
     uint32_t *counts = nullptr;
     uint64_t *bases = nullptr;
     uint64_t *total = nullptr;
@@ -269,7 +271,8 @@ int admit(vx_server *S) {
         return VX_OK;
     }
     const int n = F->n;
-    // device storage: chunks | FillWork[n] | MeshWork[n] | order[n] | ctl
+    // device storage: chunks | FillWork[n] | MeshWork[n] | order[n_items] |
+    // counts[n] | bases[n] | total, overflow | slabs[kSlabs * n] (64^3 chunks)
     F->chunk_off.resize(n);
     size_t off = 0;
     for (int i = 0; i < n; i++) {
@@ -277,13 +280,18 @@ int admit(vx_server *S) {
         off += layout_s(F->reqs[i].edge).total;
         F->max_edge = std::max(F->max_edge, F->reqs[i].edge);
     }
+    const bool slabbed = F->max_edge > 32;
+    F->n_items = n;
+    if (slabbed)
+        for (int i = 0; i < n; i++) F->n_items += F->reqs[i].edge == 64 ? kSlabs - 1 : 0;
     const size_t off_fill = off;
     const size_t off_mesh = off_fill + up256((size_t)n * sizeof(FillWork));
     const size_t off_order = off_mesh + up256((size_t)n * sizeof(MeshWork));
-    const size_t off_counts = off_order + up256((size_t)n * 4);
+    const size_t off_counts = off_order + up256((size_t)F->n_items * 4);
     const size_t off_bases = off_counts + up256((size_t)n * 4);
     const size_t off_total = off_bases + up256((size_t)n * 8);
-    const size_t bytes = off_total + 256;
+    const size_t off_slabs = off_total + 256;
+    const size_t bytes = off_slabs + (slabbed ? up256((size_t)kSlabs * n * 4) : 0);
     void *blk = nullptr;
     int st = alloc_evicting(S, bytes, &blk);
     if (st != VX_OK) {
@@ -312,6 +320,7 @@ int admit(vx_server *S) {
     F->bases = (uint64_t *)(base + off_bases);
     F->total = (uint64_t *)(base + off_total);
     F->overflow = (uint32_t *)(base + off_total + 8);
+    F->slabs = slabbed ? (uint32_t *)(base + off_slabs) : nullptr;
     Slot &sl = S->slots[slot];
     FillWork *fw = (FillWork *)sl.pinned;
     MeshWork *mw = (MeshWork *)(sl.pinned + up256((size_t)S->max_batch * sizeof(FillWork)));
@@ -380,12 +389,13 @@ int admit(vx_server *S) {
     if (e == hipSuccess)
         e = launch_fill(F->fill_dev, F->order_dev, n, F->max_edge, ctx->terrains_dev, ctx->lut_dev,
                         F->overflow + 1, s);
-    if (e == hipSuccess) e = launch_count(F->mesh_dev, n, F->max_edge, F->counts, s);
+    if (e == hipSuccess) e = launch_count(F->mesh_dev, n, F->max_edge, F->counts, F->slabs, s);
     // the scan rewrites the order array (the fill has consumed it) as the
-    // emission order, largest chunks first
+    // emission order, largest chunks first (emit items when slabbed)
     F->order_dev = (int32_t *)(base + off_order);
     if (e == hipSuccess)
-        e = launch_scan(F->counts, n, F->bases, F->total, F->overflow, F->order_dev, s);
+        e = launch_scan(F->counts, n, F->slabs ? F->mesh_dev : nullptr, F->bases, F->total,
+                        F->overflow, F->order_dev, s);
     unsigned char *back = sl.pinned + sl.pinned_bytes - 256;
     // total (8 B) | overflow | schedule error
     if (e == hipSuccess) e = hipMemcpyAsync(back, F->total, 16, hipMemcpyDeviceToHost, s);
@@ -458,8 +468,9 @@ int advance(vx_server *S, Flight *F) {
             F->out.vertices = (float *)((char *)blk + off_v);
             F->out.indices = (uint32_t *)((char *)blk + off_i);
         }
-        hipError_t e = launch_emit(F->mesh_dev, F->order_dev, F->n, F->max_edge, F->bases,
-                                   F->counts, F->out, ctx->tables_dev, F->overflow, sl.stream);
+        hipError_t e = launch_emit(F->mesh_dev, F->order_dev, F->n, F->n_items, F->max_edge,
+                                   F->bases, F->counts, F->slabs, F->out, ctx->tables_dev,
+                                   F->overflow, sl.stream);
         unsigned char *back = sl.pinned + sl.pinned_bytes - 256 - up256((size_t)S->max_batch * 12);
         if (e == hipSuccess)
             e = hipMemcpyAsync(back, F->counts, (size_t)F->n * 4, hipMemcpyDeviceToHost, sl.stream);
