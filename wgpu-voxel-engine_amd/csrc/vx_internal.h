@@ -12,8 +12,6 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kMaxOctaves = 8;     // fBm octaves supported by the noise tables
 constexpr int kDescTile = 4096;    // face descriptors staged in LDS per emit tile
 constexpr int kEmitGrid = 1280;    // emit workgroups that fill the GPU (5 per CU)
-constexpr int kEmitBlock64 = 512;  // threads per emit workgroup for 64^3 batches
-constexpr int kEmitGrid64 = 512;   // those workgroups that fill the GPU (2 per CU)
 constexpr int kFillGrid = 1024;    // fill workgroups that fill the GPU (4 per CU)
 
 // Terrain kinds (mirror of vx_terrain.kind in include/voxel.h).

@@ -543,9 +543,10 @@ __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const
 // Writes the layers y_lo <= y < y_hi of a chunk whose occupancy rows for
 // them are in S.occ: occupancy and boundary planes always, u8 ids (all solid
 // voxels = kBlockGrass) when write_ids; meta[0] = solid block id from the
-// workgroup that writes layer 0.  COHERENT: the boundary planes, which other
-// workgroups of the same launch read, are written with agent-scope stores
-// (coherent across the XCDs' L2 caches without a cache write-back).
+// workgroup that writes layer 0.  COHERENT: the occupancy rows and boundary
+// planes, which other workgroups of the same launch read, are written with
+// agent-scope stores (coherent across the XCDs' L2 caches without a cache
+// write-back; a plain store may stay in the writing XCD's L2).
 template <int N, int MAXN, bool COHERENT = false, class SH>
 __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, uint32_t solid_type,
                                     int y_lo = 0, int y_hi = N) {
@@ -556,7 +557,11 @@ __device__ void write_chunk_outputs(const FillWork &w, SH &S, bool write_ids, ui
     const int span = y_hi - y_lo;
     for (int i = y_lo * N * W + tid; i < y_hi * N * W; i += kBlock) {
         const int r = i / W, j = i - r * W;
-        w.occ[i] = occ[r * RS + j];
+        if (COHERENT)
+            __hip_atomic_store(w.occ + i, occ[r * RS + j], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        else
+            w.occ[i] = occ[r * RS + j];
     }
     // boundary planes: [0] x=0, [1] x=N-1 (bit z per y); [2] z=0, [3] z=N-1 (row bits)
     for (int k = tid; k < 4 * span * W; k += kBlock) {
@@ -816,9 +821,10 @@ struct HeavyRows {
     uint32_t n;
 };
 
-// BLOCK threads per workgroup: kBlock, or 512 for the MAXN = 64 emit, whose
-// 32 KB of u64 rows and 16 KB of row offsets allow only 2 workgroups per CU
-// (8 waves of 512 threads keep 4 waves per SIMD storing instead of 2).
+// LDS of an emit workgroup of BLOCK threads on a whole chunk (the batch
+// emit uses it up to MAXN = 32; a 64^3 chunk's 32 KB of u64 rows and 16 KB
+// of row firsts would allow only two workgroups per CU, so batches emit
+// those by slabs, and only the single-chunk emit takes MAXN = 64).
 template <int MAXN, int BLOCK = kBlock> struct MeshShared {
     typename RowType<MAXN>::T occ[MAXN * MAXN];
     typename RowType<MAXN>::T nb[4][MAXN];
@@ -1217,8 +1223,8 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
                           SH &S, int part = 0, int parts = 1, int ylo = 0, uint32_t off = 0) {
     using R = typename RowType<MAXN>::T;
     constexpr int ROWS = RowsOf<SH, N>::v;
-    // rows per thread (a 16^3 chunk in a 512-thread workgroup: half the
-    // threads own one row, the others none)
+    // rows per thread (rounded up: with more threads than rows, some
+    // threads own none)
     constexpr int RPT = (ROWS + BLOCK - 1) / BLOCK;
     const int tid = threadIdx.x;
     const bool want_v = (out.formats & kFmtVertices) != 0;
@@ -1431,12 +1437,11 @@ __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint
     }
 }
 
-// Register cap: five waves per SIMD (96 VGPRs) with 256 threads, matching
-// the five workgroups per CU that the 28 KB of LDS allows at MAXN = 32, so
-// more workgroups overlap their descriptor and store phases; four (128
-// VGPRs) for the 512-thread MAXN = 64 form, two workgroups per CU.
+// Register cap: five waves per SIMD (96 VGPRs), matching the five
+// workgroups per CU that the 29 KB of LDS allows at MAXN = 32, so more
+// workgroups overlap their descriptor and store phases.
 template <int MAXN, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 512 ? 4 : 5, 8)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8)))
 emit_kernel(const MeshWork *__restrict__ work, const int32_t *__restrict__ order,
             const uint64_t *__restrict__ bases, const uint32_t *__restrict__ counts, MeshOut out,
             const MeshTables *__restrict__ tables, uint32_t *__restrict__ overflow, int parts,
@@ -1748,18 +1753,19 @@ __global__ void __launch_bounds__(kBlock) fused_reset_kernel(FusedCtl ctl, int32
 //   mesh ticket m:  rows of m and its neighbours' planes from memory, count,
 //                   look-back for the first face, emit
 //
-// Every loop iteration a workgroup takes a fill ticket and fills it, then
-// takes a mesh ticket m (both by atomic add: a compare-and-swap claim of
-// ready tickets serialised the workgroups and ran 80 times slower).  Mesh m
-// reads the fills of chunks up to m + lag (the farthest in-batch neighbour);
-// until fill tickets that far have all been handed out, the workgroup takes
-// and runs further fill tickets itself rather than wait.  So on every CU some
-// workgroups run the VALU-bound noise while others run the store-bound emit,
-// at the fill's register budget (4 waves per SIMD) and with no halo noise.
-// Waits end: a mesh waits only for fills already handed out (fills never
-// wait) and, in the look-back, for earlier mesh tickets, whose holders are
-// running and themselves only fill or wait on earlier tickets; all waits are
-// bounded (kSpinLimit polls, then ctl.error).  Rows and planes cross
+// Tickets are taken by atomic add (a compare-and-swap claim of ready
+// tickets serialised the workgroups and ran 80 times slower).  One
+// workgroup in `ratio` meshes from the start; the others fill, and mesh once
+// the fill tickets run out.  So on every CU some workgroups run the
+// VALU-bound noise while others run the store-bound emit, at the fill's
+// register budget (4 waves per SIMD) and with no halo noise.  Mesh m reads
+// the fills of chunks up to m + lag (the farthest in-batch neighbour).
+// Waits end: fills never wait, so every fill ticket is handed out and every
+// fill publishes while fill workgroups run (the grid is at most what is
+// resident at once, and a prefix of it holds both roles); a mesh waits only
+// for fills and, in the look-back, for earlier mesh tickets, whose holders
+// are running and wait only on earlier tickets; all waits are bounded
+// (kSpinLimit polls, then ctl.error).  Rows and planes cross
 // workgroups through agent-scope loads and stores (coherent across the XCDs'
 // L2 caches without a cache write-back), the stores completed before the
 // flag is published.
@@ -1862,12 +1868,20 @@ __device__ void pipe_mesh(const FusedWork &W, int m, int n, const FusedCtl &ctl,
     __syncthreads();
 }
 
-// Register cap: four waves per SIMD, the fill kernel's budget.
+// One workgroup in kPipeRatio of the pipelined grid meshes from the start.
+constexpr int kPipeRatio = 3;
+
+// Register cap: four waves per SIMD, the fill kernel's budget.  Roles:
+// workgroup b meshes from the start when b % ratio == ratio - 1; the others
+// fill until the fill tickets run out, then mesh.  The two loops are
+// separate so that neither holds the other's values (one loop taking both
+// kinds of task kept both sets live and spilled).
 template <int MAXN>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 pipe_kernel(const FusedWork *__restrict__ work, int32_t n, int32_t lag,
             const TerrainDev *__restrict__ terrains, const GradLut *__restrict__ lut,
-            const MeshTables *__restrict__ tables, FusedCtl ctl, MeshOut out, CullJob cull) {
+            const MeshTables *__restrict__ tables, FusedCtl ctl, MeshOut out, CullJob cull,
+            int32_t ratio) {
     static_assert(MAXN <= 32, "64^3 chunks take the fused or staged form");
     __shared__ PipeShared<MAXN> S;
     const int tid = threadIdx.x;
@@ -1877,58 +1891,55 @@ pipe_kernel(const FusedWork *__restrict__ work, int32_t n, int32_t lag,
     for (int i = tid; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = tid; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     if (tid == 0) S.terrain = -1;
-    bool fills_left = true, meshes_left = true;
-    int m = -1;  // mesh ticket held, not yet ready
-    for (;;) {
-        // 1. the next fill
-        if (fills_left) {
+    const bool mesher = ratio > 1 && (int)(blockIdx.x % (uint32_t)ratio) == ratio - 1;
+    if (!mesher) {
+#pragma unroll 1
+        for (;;) {
             __syncthreads();
             if (tid == 0) S.item = (int)atomicAdd(ctl.ticket, 1u);
             __syncthreads();
             const int t = __builtin_amdgcn_readfirstlane(S.item);
-            if (t < n) {
-                const FusedWork &W = work[t];
-                const int terrain = W.f.terrain;
-                if (terrain != S.terrain) {
-                    for (int i = tid; i < 512; i += kBlock) S.perm[i] = terrains[terrain].perm[i];
-                    __syncthreads();
-                    if (tid == 0) S.terrain = terrain;
-                }
-                const TerrainDev &T = terrains[terrain];
-                if (W.f.edge == 16) pipe_fill<16, MAXN>(W.f, T, ctl.filled + t, S);
-                else if (MAXN >= 32) pipe_fill<(MAXN >= 32 ? 32 : 16), MAXN>(W.f, T, ctl.filled + t, S);
-            } else {
-                fills_left = false;
+            if (t >= n) break;
+            const FusedWork &W = work[t];
+            const int terrain = W.f.terrain;
+            if (terrain != S.terrain) {
+                for (int i = tid; i < 512; i += kBlock) S.perm[i] = terrains[terrain].perm[i];
+                __syncthreads();
+                if (tid == 0) S.terrain = terrain;
             }
+            const TerrainDev &T = terrains[terrain];
+            if (W.f.edge == 16) pipe_fill<16, MAXN>(W.f, T, ctl.filled + t, S);
+            else if (MAXN >= 32) pipe_fill<(MAXN >= 32 ? 32 : 16), MAXN>(W.f, T, ctl.filled + t, S);
         }
-        // 2. a mesh ticket, unless one is held (an atomic add: a
-        // compare-and-swap claim of ready tickets serialised the workgroups)
+    }
+    // Meshes in ticket order.  Mesh m needs the fills of m and of its
+    // in-batch neighbours (at most m + lag): the workgroup first waits until
+    // fill tickets that far have been handed out, then for those fills.
+#pragma unroll 1
+    for (;;) {
         __syncthreads();
         if (tid == 0) {
-            int mm = m;
-            if (mm < 0 && meshes_left) mm = (int)atomicAdd(ctl.mesh_ticket, 1u);
-            // 3. ready once the fills of chunks up to mm + lag are all out
-            const uint32_t f = __hip_atomic_load(ctl.ticket, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const int mm = (int)atomicAdd(ctl.mesh_ticket, 1u);
             S.mitem = mm;
-            S.item = mm >= 0 && mm < n && (f >= (uint32_t)n || f > (uint32_t)mm + (uint32_t)lag);
+            if (mm < n) {
+                const uint32_t need = (uint32_t)min((int64_t)n, (int64_t)mm + lag + 1);
+                uint32_t spins = 0;
+                while (__hip_atomic_load(ctl.ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       need) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(ctl.error, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
         }
         __syncthreads();
-        m = __builtin_amdgcn_readfirstlane(S.mitem);
-        const bool ready = __builtin_amdgcn_readfirstlane(S.item) != 0;
-        if (m >= n) {  // every mesh ticket is taken
-            meshes_left = false;
-            m = -1;
-        }
-        if (ready) {
-            const FusedWork &W = work[m];
-            if (W.f.edge == 16) pipe_mesh<16, MAXN>(W, m, n, ctl, out, S);
-            else if (MAXN >= 32) pipe_mesh<(MAXN >= 32 ? 32 : 16), MAXN>(W, m, n, ctl, out, S);
-            m = -1;
-        }
-        // a held mesh that is not ready yet waits for further fills, which
-        // this workgroup takes itself in the next iteration
-        if (!fills_left && !meshes_left && m < 0) break;
+        const int m = __builtin_amdgcn_readfirstlane(S.mitem);
+        if (m >= n) break;
+        const FusedWork &W = work[m];
+        if (W.f.edge == 16) pipe_mesh<16, MAXN>(W, m, n, ctl, out, S);
+        else if (MAXN >= 32) pipe_mesh<(MAXN >= 32 ? 32 : 16), MAXN>(W, m, n, ctl, out, S);
     }
 }
 
@@ -2171,10 +2182,10 @@ hipError_t launch_pipe(const FusedWork *work, int32_t n, int32_t max_edge, int32
     fused_reset_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(ctl, n);
     if (max_edge <= 16)
         pipe_kernel<16><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, lag, terrains, lut, tables,
-                                                               ctl, out, cull);
+                                                               ctl, out, cull, kPipeRatio);
     else
         pipe_kernel<32><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, lag, terrains, lut, tables,
-                                                               ctl, out, cull);
+                                                               ctl, out, cull, kPipeRatio);
     return hipGetLastError();
 }
 
