@@ -318,10 +318,10 @@ __device__ void density_lines(const TerrainDev &T, int ox, int oy, int oz, const
 // octaves known at compile time: the fBm sum of a voxel is one register (no
 // indexed register moves), the occupancy ballot follows each row, and the
 // per-octave rehash is taken from the row's change mask (one uniform LDS
-// word per row instead of one readfirstlane per octave).  Per-octave column
-// terms (fade weights, x permutation entries, z cell) and the 8 corner
-// coefficients stay in registers; the lattice fractions are recomputed in
-// the rare rehash.  Octaves are summed in the same order as the oracle.
+// word per row instead of one readfirstlane per octave).  Per-octave fade
+// weights of the column and the 8 corner coefficients stay in registers;
+// the column's lattice cells and fractions are recomputed in the rare
+// rehash.  Octaves are summed in the same order as the oracle.
 template <int N, int MAXN, bool DBG, int SEG, int OCT, bool REF, class SH>
 __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, const Halo &H,
                                   SH &S, float *noise_out, int seg_only = -1, int g_first = 0) {
@@ -358,19 +358,12 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
         asm volatile("" : "+v"(lane_i));
         line_column<N>(li, lane_i, H, ox, oz, wxi, wzi);
         float u[OCT], wz[OCT], c[OCT][8], gy[OCT][8];
-        int pX0[OCT], pX1[OCT], Zi[OCT];
 #pragma unroll
         for (int o = 0; o < OCT; o++) {
             const float X = (float)wxi * fo[o];
-            const float fxl = floorf(X);
-            u[o] = fade(X - fxl);
-            const int Xi = cell8(fxl);
-            pX0[o] = S.perm[Xi];
-            pX1[o] = S.perm[Xi + 1];
+            u[o] = fade(X - floorf(X));
             const float Z = (float)wzi * fo[o];
-            const float fzl = floorf(Z);
-            wz[o] = fade(Z - fzl);
-            Zi[o] = cell8(fzl);
+            wz[o] = fade(Z - floorf(Z));
         }
         RowKeep<N> K;
 #pragma unroll 1
@@ -386,12 +379,13 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
                 float f = T.frequency;
                 for (int k = 0; k < o; k++) f = f * T.lacunarity;
                 const float X = (float)wxi * f, Z = (float)wzi * f;
-                const float fx = X - floorf(X), fz = Z - floorf(Z);
+                const float fxl = floorf(X), fzl = floorf(Z);
+                const float fx = X - fxl, fz = Z - fzl;
                 const float fx1 = fx - 1.0f, fz1 = fz - 1.0f;
-                int px0 = pX0[0], px1 = pX1[0], zi = Zi[0];
-#pragma unroll
-                for (int q = 1; q < OCT; q++)
-                    if (o == q) px0 = pX0[q], px1 = pX1[q], zi = Zi[q];
+                // the column's x and z cells, recomputed here rather than
+                // kept per octave in registers (12 VGPRs at four octaves)
+                const int Xi = cell8(fxl), zi = cell8(fzl);
+                const int px0 = S.perm[Xi], px1 = S.perm[Xi + 1];
                 int h[8];
                 corner_hashes<REF>(S, px0, px1, Yi, zi, h);
                 float cn[8], gn[8];
