@@ -246,6 +246,7 @@ def config3_batch(ctx):
     coords3 = [(i % 64, 0, i // 64) for i in range(n3)]
     b3 = V.Batch(ctx, coords3, edges, [tids[(int(k), int(e))] for e, k in zip(edges, cls)],
                  np.full((n3, 4), -1, np.int32), formats=V.FMT_VERTICES)
+    b3.edges = edges
     return b3, n3
 
 
@@ -276,6 +277,16 @@ def configs_leg(ctx, reps: int = 20):
         ms = (time.perf_counter() - t0) * 1e3 / reps
         out[name] = {"chunks": n, "faces": faces, "ms_per_batch": round(ms, 4),
                      "chunks_per_s": round(n / (ms * 1e-3), 1)}
+        if name == "config3":
+            # stage times (HIP events); the emit's algorithmic bytes: 104 B per
+            # face written + every chunk's occupancy rows read (no neighbours)
+            p = b.profile(reps)
+            rows = int(sum(int(e) * int(e) * ((int(e) + 31) // 32) * 4 for e in b.edges))
+            emit_bytes = faces * 104 + rows
+            out[name]["stages_ms"] = {k: round(float(v), 4) for k, v in
+                                      zip(("fill", "count", "scan", "emit"), p[:4])}
+            out[name]["emit_algorithmic_bytes"] = emit_bytes
+            out[name]["emit_GB/s"] = round(emit_bytes / (float(p[3]) * 1e-3) / 1e9, 1)
         b.close()
     return out
 
@@ -437,7 +448,7 @@ def main():
             b.set_mode({"fused": V.BATCH_FUSED, "pipelined": V.BATCH_PIPELINED}.get(
                 args.pmc_run, V.BATCH_STAGED))
             if args.pmc_run == "pipelined":
-                b.set_grid(4 * 256)
+                b.set_grid(8 * 256)
         elif args.pmc_run == "config3":
             b, _ = config3_batch(ctx)
             b.set_mode(V.BATCH_STAGED)
@@ -506,8 +517,10 @@ def main():
         if m == mode:
             continue
         b.set_mode(m)
-        if m == V.BATCH_PIPELINED:
-            b.set_grid(4 * 256)
+        # a persistent form at the autotuner's fastest grid for it
+        grids = [c["grid"] for c in sorted(tune.get("candidates", []), key=lambda c: c["ms"])
+                 if c["mode"] == names_of[m]]
+        b.set_grid(grids[0] if grids else 0)
         b.run()
         assert b.wait() == total_faces
         oe = timed(reps2, 1)

@@ -290,12 +290,14 @@ int vx_batch_destroy(vx_batch *batch);
  *                    single-pass device-wide scan.  Eligible when every in-batch
  *                    neighbour is the adjacent chunk with the same terrain.
  *   VX_BATCH_STAGED  four kernels: fill -> count -> scan -> emit
- *   VX_BATCH_PIPELINED  one persistent kernel whose workgroups each fill their
- *                    next chunk before meshing their previous one, so noise
- *                    (VALU) and emission (HBM stores) run side by side on
- *                    every CU; neighbour planes come from the neighbours'
- *                    fills (recomputed only if one has not finished).
- *                    Eligible as VX_BATCH_FUSED, for edges up to 32.
+ *   VX_BATCH_PIPELINED  one kernel with fill and mesh roles: one workgroup in
+ *                    three meshes chunks in batch order (count, single-pass
+ *                    scan, emit) as their fills and their neighbours' fills
+ *                    publish; the others fill, then mesh once every fill is
+ *                    handed out, so noise (VALU) and emission (HBM stores)
+ *                    run side by side on every CU; neighbour planes come
+ *                    from the neighbours' fills.  Eligible as VX_BATCH_FUSED,
+ *                    for edges up to 32.
  * Every form produces identical bytes. */
 #define VX_BATCH_AUTO 0
 #define VX_BATCH_FUSED 1
@@ -337,9 +339,9 @@ int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
 /* Autotuner: times every candidate execution form and launch shape of this
  * batch (staged; fused with one workgroup per chunk; fused with 2..4
  * persistent workgroups per CU; pipelined with one workgroup per chunk and
- * with 3 or 4 persistent workgroups per CU) over `reps` graph replays each,
- * keeps the
- * fastest for the batch, and caches the choice for batches of the same shape
+ * with 4 or 8 workgroups per CU) over `reps` graph replays each, in two
+ * rounds (each candidate's faster round counts), keeps the fastest for the
+ * batch, and caches the choice for batches of the same shape
  * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
  * VX_TUNE_CACHE set to a file path the cache persists across processes.
  * Batches created later in VX_BATCH_AUTO use a cached choice. */
