@@ -267,8 +267,8 @@ def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES 
 
 # fused with one workgroup per chunk; fused with 3 persistent workgroups that
 # take every chunk ticket in turn; staged; pipelined with one workgroup per
-# chunk, with 3 persistent workgroups, and with a single workgroup (which
-# must fill ahead until each mesh is ready, then drain the meshes)
+# chunk, with 3 workgroups (two fill, one meshes from the start), and with a
+# single workgroup (a filler: it fills every chunk, then meshes them all)
 MODES = [pytest.param((V.BATCH_FUSED, 0), id="fused"),
          pytest.param((V.BATCH_FUSED, 3), id="fused_persistent3"),
          pytest.param((V.BATCH_STAGED, 0), id="staged"),
@@ -429,6 +429,61 @@ def test_batch_ragged(ctx, oracle_mod, mode):
                                       frequency=1 / 32, height_scale=float(e), height_max=int(e) - 1))
     coords = [(i % 8, 0, i // 8) for i in range(n)]
     batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1), mode=mode)
+
+
+def test_batch_fill_counts_against_resident_neighbours(ctx, oracle_mod):
+    """1,024 chunks of 16^3 (one fill workgroup each) with no neighbour
+    inside the batch: the staged fill counts the faces itself, so the batch
+    has no count kernel.  The first column's -x neighbours are resident
+    chunks of an earlier batch, every other side is absent.  Ids, counts,
+    bases and every mesh equal the oracle's; a replay gives the same; the
+    per-chunk path (count kernel) agrees."""
+    O = oracle_mod
+    say("batch fill counts")
+    t = V.Terrain(kind=V.TERRAIN_DENSITY, seed=7, octaves=4, frequency=1 / 64, lacunarity=2.0,
+                  gain=0.5, y_scale=1 / 128)
+    tid = ctx.terrain(t)
+    side = 32
+    res = V.Batch(ctx, [(0, 0, z) for z in range(side)], 16, tid, np.full((side, 4), -1),
+                  formats=V.FMT_VERTICES)
+    res.run()
+    res.wait()
+    ext = [res.chunk(z) for z in range(side)]
+    coords = [(x, 0, z) for x in range(1, side + 1) for z in range(side)]
+    nb = np.full((len(coords), 4), -1, np.int32)
+    for i, (x, _, z) in enumerate(coords):
+        if x == 1:
+            nb[i, 0] = -2 - z
+    b = V.Batch(ctx, coords, 16, tid, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES, ext=ext)
+    b.set_mode(V.BATCH_STAGED)
+    ot = oterrain(O, t)
+    ids = {c: O.chunk_fill(ot, *c, 16) for c in coords}
+    ids.update({(0, 0, z): O.chunk_fill(ot, 0, 0, z, 16) for z in range(side)})
+    for run in range(3):
+        if run < 2:
+            b.run()
+        else:
+            b.run_per_chunk()
+        total = b.wait()
+        verts, idxs, insts = b.mesh(V.FMT_VERTICES), b.mesh(V.FMT_INDICES), b.mesh(V.FMT_INSTANCES)
+        first_expected = 0
+        for i, c in enumerate(coords):
+            if run == 0:
+                assert (b.ids(i) == ids[c]).all(), i
+            adj = [ids[(0, 0, c[2])] if c[0] == 1 else None, None, None, None]
+            faces, inst, vert, ix = O.mesh_buffers(ids[c], 16, (c[0] * 16, 0, c[2] * 16), adj)
+            first, cnt = b.chunk_faces(i)
+            assert (first, cnt) == (first_expected, len(faces)), (run, i)
+            if run == 0 or i % 97 == 0:
+                assert (bits(verts[4 * first:4 * (first + cnt)]) == bits(vert)).all(), (run, i)
+                assert (idxs[6 * first:6 * (first + cnt)] == ix).all(), (run, i)
+                assert (bits(insts[first:first + cnt]) == bits(inst)).all(), (run, i)
+            first_expected += cnt
+        assert total == first_expected
+    b.close()
+    for c in ext:
+        c.release()
+    res.close()
 
 
 @pytest.mark.parametrize("mode", MODES)

@@ -202,7 +202,7 @@ hipError_t enqueue_all(vx_batch *b) {
     if (use_fused(b)) return enqueue_fused(b);
     hipError_t e = launch_cull(b->cull, b->ctx->stream);
     if (e == hipSuccess) e = enqueue_fill(b);
-    if (e == hipSuccess) e = enqueue_count(b);
+    if (e == hipSuccess && !b->fill_counts) e = enqueue_count(b);
     if (e == hipSuccess) e = enqueue_scan(b);
     if (e == hipSuccess) e = enqueue_emit(b);
     return e;
@@ -433,6 +433,20 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
         }
     }
     // fused form: every in-batch neighbour must be the adjacent chunk with the
+    // The fill counts the faces itself when every chunk has its own
+    // workgroup and no neighbour inside the batch (each neighbour plane is
+    // absent or resident before the run): the staged form then has no
+    // count kernel.
+    b->fill_counts = fill_whole_chunks(n);
+    for (int i = 0; i < n && b->fill_counts; i++)
+        for (int k = 0; k < 4; k++)
+            if (reqs[i].nb[k] >= 0) b->fill_counts = false;
+    if (b->fill_counts)
+        for (int i = 0; i < n; i++) {
+            for (int k = 0; k < 4; k++) fw[i].cnb[k] = mw[i].nb[k];
+            fw[i].count_out = b->counts + i;
+            fw[i].slab_out = b->slabs ? b->slabs + (size_t)kSlabs * i : nullptr;
+        }
     // same terrain, so its facing plane can be recomputed from noise; the
     // reference-rule terrain is generated by the fill kernels only
     static const int ddx[4] = {-1, 1, 0, 0}, ddz[4] = {0, 0, -1, 1};
@@ -925,7 +939,7 @@ extern "C" int vx_batch_profile(vx_ctx *ctx, vx_batch *b, int32_t reps, float ms
         hipError_t e = hipEventRecord(ev[0], ctx->stream);
         if (e == hipSuccess) e = enqueue_fill(b);
         if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
-        if (e == hipSuccess) e = enqueue_count(b);
+        if (e == hipSuccess && !b->fill_counts) e = enqueue_count(b);
         if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
         if (e == hipSuccess) e = enqueue_scan(b);
         if (e == hipSuccess) e = hipEventRecord(ev[3], ctx->stream);

@@ -158,6 +158,7 @@ struct vx_batch {
     int32_t tuned_mode = 0, tuned_grid = 0;
     bool fused_ok = false;          // fused form eligible
     bool pipe_ok = false;           // pipelined form eligible (fused_ok, edges <= 32)
+    bool fill_counts = false;       // the staged fill also counts (no count kernel)
     int32_t pipe_lag = 0;           // farthest in-batch neighbour index ahead of a chunk
     vx::FusedWork *fused_dev = nullptr;
     void *fctl = nullptr;           // ticket | error | status[n]

@@ -71,6 +71,14 @@ struct FillWork {
     uint32_t *planes;      // 4 planes * edge * W words: x=0, x=N-1, z=0, z=N-1
     uint32_t *meta;        // [0] solid block id (or kBlockMixed)
     float *noise_out;      // optional raw noise (density: edge^3, heightmap: edge^2)
+    // Optional (fill_kernel, one workgroup per chunk): the fill also counts
+    // the chunk's visible faces, for chunks whose every neighbour is absent
+    // or was written before the launch.  cnb[k] = the plane of neighbour k
+    // (-x, +x, -z, +z) facing this chunk, null = absent; count_out[0] = the
+    // face count; slab_out[0..kSlabs) = a 64^3 chunk's per-slab counts.
+    const uint32_t *cnb[4];
+    uint32_t *count_out;   // null: the count kernel counts
+    uint32_t *slab_out;
 };
 
 // One chunk to mesh.  nb[k] points at the neighbour's plane facing this
@@ -159,6 +167,9 @@ inline int words_per_row(int edge) { return (edge + 31) / 32; }
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const TerrainDev *terrains, const GradLut *lut, uint32_t *error,
                        hipStream_t s);
+// True iff launch_fill gives each chunk one workgroup (fill_kernel), so that
+// FillWork.count_out is honoured for every chunk of the batch.
+inline bool fill_whole_chunks(int32_t n) { return n >= kFillGrid; }
 // Batches that hold 64^3 chunks emit by items: each 64^3 chunk is kSlabs
 // y-slabs of kSlabLayers layers (face counts per slab in slabs[kSlabs * i
 // ..], written by the count kernel), any other chunk one item.  With `items`

@@ -613,6 +613,12 @@ __device__ void fill_part(const FillWork &w, const TerrainDev &T, FillShared<MAX
                                  (part + 1) * (N / PARTS));
 }
 
+template <int N, int MAXN, class SH>
+__device__ void count_loaded(SH &S, uint32_t *count_out, uint32_t *slabs);
+
+// With w.count_out the fill also counts the chunk's faces from the rows it
+// holds in LDS and the neighbour planes w.cnb (all written before the
+// launch), which spares the batch its count kernel.
 template <int N, int MAXN, bool DBG>
 __device__ __forceinline__ void fill_chunk(const FillWork &w, const TerrainDev &T, FillShared<MAXN> &S) {
     const Halo H{0, 0u};
@@ -620,6 +626,22 @@ __device__ __forceinline__ void fill_chunk(const FillWork &w, const TerrainDev &
                                                          w.noise_out);
     __syncthreads();
     write_chunk_outputs<N, MAXN>(w, S, true, kBlockGrass);
+    if (w.count_out) {
+        using R = typename RowType<MAXN>::T;
+        constexpr int W = N == 64 ? 2 : 1;
+        for (int i = threadIdx.x; i < 4 * N; i += kBlock) {  // S.nb: no halo lines here
+            const int k = i / N, y = i - k * N;
+            const uint32_t *p = w.cnb[k];
+            R v = 0;
+            if (p) {
+                if (W == 2) v = (R)p[2 * y] | ((R)p[2 * y + 1] << (W == 2 ? 32 : 0));
+                else v = (R)p[y];
+            }
+            S.nb[k][y] = v;
+        }
+        __syncthreads();
+        count_loaded<N, MAXN>(S, w.count_out, w.slab_out);
+    }
 }
 
 template <int N, int MAXN, bool REF>
@@ -969,14 +991,12 @@ template <int MAXN> struct CountShared {
     uint32_t red[kBlock / 64];
 };
 
-// A 64^3 chunk's count also goes out per slab (slabs[0..kSlabs), when
-// given): thread t's rows t + kBlock * i lie in slab i / (kSlabLayers * 64 /
-// kBlock).
-template <int N, int MAXN>
-__device__ void count_chunk(const MeshWork &w, CountShared<MAXN> &S, uint32_t *count_out,
-                            uint32_t *slabs) {
-    load_rows<N, MAXN>(w, S);
-    __syncthreads();
+// Counts the faces of the chunk whose rows and neighbour planes are in
+// LDS (S.occ, S.nb).  A 64^3 chunk's count also goes out per slab
+// (slabs[0..kSlabs), when given): thread t's rows t + kBlock * i lie in
+// slab i / (kSlabLayers * 64 / kBlock).
+template <int N, int MAXN, class SH>
+__device__ void count_loaded(SH &S, uint32_t *count_out, uint32_t *slabs) {
     if constexpr (N == 64) {
         using R = typename RowType<MAXN>::T;
         constexpr int PER = kSlabLayers * 64 / kBlock;  // a thread's rows per slab
@@ -1013,6 +1033,14 @@ __device__ void count_chunk(const MeshWork &w, CountShared<MAXN> &S, uint32_t *c
         const uint32_t total = count_rows<N, MAXN>(S);
         if (threadIdx.x == 0) *count_out = total;
     }
+}
+
+template <int N, int MAXN>
+__device__ void count_chunk(const MeshWork &w, CountShared<MAXN> &S, uint32_t *count_out,
+                            uint32_t *slabs) {
+    load_rows<N, MAXN>(w, S);
+    __syncthreads();
+    count_loaded<N, MAXN>(S, count_out, slabs);
 }
 
 template <int MAXN>
