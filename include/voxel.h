@@ -428,7 +428,11 @@ int vx_server_drain(vx_server *server);
  * caller (either output may be NULL); VX_ERR_NOT_READY when not resident.
  * The cache keys chunks by coordinate, edge and terrain: requests for one
  * coordinate under two terrains are two chunks, and a neighbour snapshot
- * (VX_NB_RESIDENT) only takes neighbours of the request's own terrain. */
+ * (VX_NB_RESIDENT) only takes neighbours of the request's own terrain.
+ * A completed request's entry is not evicted before the second pump after
+ * the one (or the submit) that completed it, so a caller that polls and
+ * looks up its completions between pumps always finds them; under a budget
+ * that binds, later admissions wait for that instead. */
 int vx_server_lookup(vx_server *server, vx_coord coord, int32_t edge, int32_t terrain_id,
                      vx_chunk **chunk, vx_mesh **mesh);
 int vx_server_stats_get(vx_server *server, vx_server_stats *stats);

@@ -86,6 +86,7 @@ struct Entry {
     vx_chunk *chunk = nullptr;
     vx_mesh *mesh = nullptr;
     std::list<Key>::iterator lru;
+    uint64_t born = 0;  // pump epoch of its latest completion (miss or hit)
 };
 
 struct Waiter {
@@ -143,6 +144,11 @@ struct vx_server {
     std::vector<Slot> slots;
     std::vector<vx_completion> done;
     vx_server_stats st{};
+    // Pump epoch: pump_once advances it.  An entry completed in this epoch
+    // or the previous one is not evicted, so a client that polls and looks
+    // up its completions between pumps finds them resident (admission of a
+    // later flight in the same pump would otherwise evict them first).
+    uint64_t epoch = 2;
 };
 
 namespace {
@@ -187,14 +193,33 @@ void evict_entry(vx_server *S, const Key &k) {
     S->st.evictions++;
 }
 
+bool pinned(const vx_server *S, const Entry &e) { return S->epoch - e.born < 2; }
+
+void mark_completed(vx_server *S, Entry &e) { e.born = S->epoch; }
+
 // Allocates `bytes` from the context heap, evicting least-recently-used
-// cache entries while the HBM budget refuses it.
-int alloc_evicting(vx_server *S, size_t bytes, void **out) {
+// cache entries while the HBM budget refuses it; entries completed in this
+// pump or the previous one are kept.  On VX_ERR_BUDGET, *pinned_left tells
+// whether kept entries remain (memory a later pump can free).
+int alloc_evicting(vx_server *S, size_t bytes, void **out, bool *pinned_left) {
+    *pinned_left = false;
     for (;;) {
         int st = S->ctx->dev_alloc(bytes, out);
         if (st != VX_ERR_BUDGET) return st;
-        if (S->lru.empty()) return VX_ERR_BUDGET;
-        evict_entry(S, S->lru.back());
+        auto it = S->lru.end();
+        bool found = false;
+        while (it != S->lru.begin()) {
+            --it;
+            if (!pinned(S, S->cache.at(*it))) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) {
+            *pinned_left = !S->lru.empty();
+            return VX_ERR_BUDGET;
+        }
+        evict_entry(S, *it);
     }
 }
 
@@ -250,6 +275,7 @@ int admit(vx_server *S) {
         auto hit = S->cache.find(k);
         if (hit != S->cache.end()) {
             S->lru.splice(S->lru.begin(), S->lru, hit->second.lru);
+            mark_completed(S, hit->second);
             complete(S, ticket, r, VX_OK, true, hit->second.mesh->faces, r.arrival_ns, now_ns(),
                      nullptr);
             continue;
@@ -293,11 +319,12 @@ int admit(vx_server *S) {
     const size_t off_slabs = off_total + 256;
     const size_t bytes = off_slabs + (slabbed ? up256((size_t)kSlabs * n * 4) : 0);
     void *blk = nullptr;
-    int st = alloc_evicting(S, bytes, &blk);
+    bool pinned_left = false;
+    int st = alloc_evicting(S, bytes, &blk, &pinned_left);
     if (st != VX_OK) {
-        // nothing fits now: put the requests back unless no flight can ever
-        // free memory, in which case they fail
-        if (S->flights.empty()) {
+        // nothing fits now: put the requests back unless neither a flight
+        // nor a later pump can free memory, in which case they fail
+        if (S->flights.empty() && !pinned_left) {
             fail_flight_requests(S, F, st);
         } else {
             for (int i = n - 1; i >= 0; i--) {
@@ -450,10 +477,12 @@ int advance(vx_server *S, Flight *F) {
         }
         if (bytes == 0) bytes = 256;
         void *blk = nullptr;
-        int st = alloc_evicting(S, bytes, &blk);
+        bool pinned_left = false;
+        int st = alloc_evicting(S, bytes, &blk, &pinned_left);
         // Younger flights wait for older ones to retire and free memory; the
-        // oldest flight never waits (it would wait on flights waiting on it).
-        if (st == VX_ERR_BUDGET && S->flights.front() != F) return VX_OK;
+        // oldest flight waits only for entries a later pump can evict (it
+        // would otherwise wait on flights waiting on it).
+        if (st == VX_ERR_BUDGET && (S->flights.front() != F || pinned_left)) return VX_OK;
         if (st != VX_OK) {
             fail_flight_requests(S, F, st);
             release_flight(S, F);
@@ -544,6 +573,7 @@ int advance(vx_server *S, Flight *F) {
         en.chunk = ch;
         en.mesh = m;
         en.lru = S->lru.begin();
+        en.born = S->epoch;
         S->cache[k] = en;
         S->inflight_keys.erase(k);
         for (const Waiter &w : F->waiters[i])
@@ -556,6 +586,7 @@ int advance(vx_server *S, Flight *F) {
 
 int pump_once(vx_server *S) {
     int rc = VX_OK;
+    S->epoch++;
     // retire / advance in admission order
     std::vector<Flight *> fl = S->flights;
     for (Flight *F : fl) {
@@ -655,6 +686,7 @@ extern "C" int vx_server_submit(vx_server *S, const vx_request *reqs, int32_t n,
         auto hit = S->cache.find(k);
         if (hit != S->cache.end()) {  // resident: served without device work
             S->lru.splice(S->lru.begin(), S->lru, hit->second.lru);
+            mark_completed(S, hit->second);
             complete(S, ticket, r, VX_OK, true, hit->second.mesh->faces, r.arrival_ns, now_ns(),
                      nullptr);
             continue;
