@@ -226,11 +226,12 @@ void make_mesh_tables(MeshTables &T) {
                     acc = acc + Trow[3] * H[3][c];
                     T.Mb[d][s][r][c] = acc;
                 }
+                T.Mb[d][s][r][3] = 0.0f;
                 for (int q = 0; q < 4; q++) {
                     float acc = T.Mb[d][s][r][0] * corner[q][0];
                     acc = acc + T.Mb[d][s][r][1] * corner[q][1];
                     acc = acc + T.Mb[d][s][r][2] * corner[q][2];
-                    T.P[d][s][q][r] = acc;
+                    T.P[d][r][s][q] = acc;
                 }
             }
         }

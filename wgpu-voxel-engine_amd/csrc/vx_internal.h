@@ -52,12 +52,15 @@ struct GradLut {
 
 // Meshing constants computed on the host from the rotation of each face
 // direction (nalgebra from_euler_angles restated) and the translation sign:
-//   P[d][s][corner][r] = ((M(r,0)*cx + M(r,1)*cy) + M(r,2)*cz)
+//   P[d][r][s][corner] = ((M(r,0)*cx + M(r,1)*cy) + M(r,2)*cz)
 //   Mb[d][s][r][c]     = M(r,c) of T*R for a translation component of sign s
-// where s = 1 iff that translation component is negative.
+//                        (c = 3 unused, zero)
+// where s = 1 iff that translation component is negative.  The innermost
+// four floats of each are what one face reads for one coordinate r: a
+// single 16-byte LDS read.
 struct MeshTables {
-    float P[6][2][4][3];
-    float Mb[6][2][3][3];
+    alignas(16) float P[6][3][2][4];
+    alignas(16) float Mb[6][2][3][4];
     uint32_t tex[3][6];  // [block id][dir] -> texture layer (row 0 unused)
 };
 

@@ -848,8 +848,8 @@ template <int MAXN, int BLOCK = kBlock> struct MeshShared {
     // first face of every row, for the interleaved descriptor walk (4 KB at
     // MAXN = 32: 28 KB in all, 5 workgroups per CU; 16 KB at MAXN = 64)
     uint32_t rowfirst[MAXN >= 32 ? MAXN * MAXN : 1];
-    float P[6][2][4][3];
-    float Mb[6][2][3][3];
+    alignas(16) float P[6][3][2][4];
+    alignas(16) float Mb[6][2][3][4];
     uint32_t tex[3][6];
     uint32_t red[BLOCK / 64];
     HeavyRows heavy;
@@ -870,8 +870,8 @@ struct SlabShared {
     uint64_t nb[4][kSlabLayers];
     uint32_t desc[kDescTile / 2];
     uint32_t rowfirst[kSlabLayers * 64];
-    float P[6][2][4][3];
-    float Mb[6][2][3][3];
+    alignas(16) float P[6][3][2][4];
+    alignas(16) float Mb[6][2][3][4];
     uint32_t tex[3][6];
     uint32_t red[kBlock / 64];
     HeavyRows heavy;
@@ -1369,9 +1369,15 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
             if (want_v) {
                 float v[20];
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                for (int k = 0; k < 3; k++) {  // one 16-B LDS read per coordinate
+                    const float4 pk = *reinterpret_cast<const float4 *>(&S.P[d][k][sg[k]][0]);
+                    v[0 * 5 + k] = pk.x + t[k];
+                    v[1 * 5 + k] = pk.y + t[k];
+                    v[2 * 5 + k] = pk.z + t[k];
+                    v[3 * 5 + k] = pk.w + t[k];
+                }
 #pragma unroll
-                    for (int k = 0; k < 3; k++) v[c * 5 + k] = S.P[d][sg[k]][c][k] + t[k];
+                for (int c = 0; c < 4; c++) {
                     v[c * 5 + 3] = (c & 1) ? 1.0f : 0.0f;
                     v[c * 5 + 4] = (c & 2) ? 1.0f : 0.0f;
                 }
@@ -1405,11 +1411,14 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
                     if (valid && half == h) {
                         uint32_t *rec = stw + hl * 17;
 #pragma unroll
-                        for (int c = 0; c < 3; c++) {
-#pragma unroll
-                            for (int k = 0; k < 3; k++) rec[c * 4 + k] = __float_as_uint(S.Mb[d][sg[k]][k][c]);
-                            rec[c * 4 + 3] = 0u;  // M(3,c) = +0
+                        for (int k = 0; k < 3; k++) {  // row k of T*R for its sign
+                            const float4 mk = *reinterpret_cast<const float4 *>(&S.Mb[d][sg[k]][k][0]);
+                            rec[0 * 4 + k] = __float_as_uint(mk.x);
+                            rec[1 * 4 + k] = __float_as_uint(mk.y);
+                            rec[2 * 4 + k] = __float_as_uint(mk.z);
                         }
+#pragma unroll
+                        for (int c = 0; c < 3; c++) rec[c * 4 + 3] = 0u;  // M(3,c) = +0
 #pragma unroll
                         for (int k = 0; k < 3; k++) rec[12 + k] = __float_as_uint(t[k]);
                         rec[15] = 0x3F800000u;  // 1.0f
@@ -1448,8 +1457,8 @@ __device__ __forceinline__ void emit_item(const MeshWork &w, uint64_t base, uint
         return;
     }
     if (F == 0) return;  // an air chunk (or one with no visible face)
-    for (int i = threadIdx.x; i < 6 * 2 * 4 * 3; i += BLOCK) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
-    for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += BLOCK) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 3 * 2 * 4; i += BLOCK) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 2 * 3 * 4; i += BLOCK) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = threadIdx.x; i < 18; i += BLOCK) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     switch (w.edge) {
     case 16: emit_chunk<16, MAXN, BLOCK>(w, base, F, out, S, part, parts); break;
@@ -1490,8 +1499,8 @@ __device__ void emit_slab_item(const MeshWork &w, uint64_t chunk_base, const uin
         return;
     }
     if (F == 0) return;
-    for (int i = threadIdx.x; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
-    for (int i = threadIdx.x; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 3 * 2 * 4; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = threadIdx.x; i < 6 * 2 * 3 * 4; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = threadIdx.x; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     load_slab(w, S, slab * kSlabLayers);
     __syncthreads();
@@ -1559,8 +1568,8 @@ template <int MAXN> struct FusedShared {
     typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t desc[kDescTile];
     uint32_t rowfirst[MAXN == 32 ? MAXN * MAXN : 1];  // interleaved descriptor walk
-    float P[6][2][4][3];
-    float Mb[6][2][3][3];
+    alignas(16) float P[6][3][2][4];
+    alignas(16) float Mb[6][2][3][4];
     uint32_t tex[3][6];
     uint32_t red[kBlock / 64];
     float4 stage[kBlock / 64][kStageWords / 4];
@@ -1725,8 +1734,8 @@ fused_kernel(const FusedWork *__restrict__ work, int32_t n,
     const int tid = threadIdx.x;
     if (cull.n > 0) cull_slice(cull);
     for (int i = tid; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
-    for (int i = tid; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
-    for (int i = tid; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = tid; i < 6 * 3 * 2 * 4; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = tid; i < 6 * 2 * 3 * 4; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = tid; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     if (tid == 0) S.terrain = -1;
     for (;;) {
@@ -1800,8 +1809,8 @@ template <int MAXN> struct PipeShared {
     typename RowType<MAXN>::T nb[4][MAXN];
     uint32_t desc[kDescTile / 2];
     uint32_t rowfirst[MAXN * MAXN];  // interleaved descriptor walk
-    float P[6][2][4][3];
-    float Mb[6][2][3][3];
+    alignas(16) float P[6][3][2][4];
+    alignas(16) float Mb[6][2][3][4];
     uint32_t tex[3][6];
     uint32_t red[kBlock / 64];
     float4 stage[kBlock / 64][kStageWords / 4];
@@ -1909,8 +1918,8 @@ pipe_kernel(const FusedWork *__restrict__ work, int32_t n, int32_t lag,
     const int tid = threadIdx.x;
     if (cull.n > 0) cull_slice(cull);
     for (int i = tid; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = lut->c[i >> 4][i & 15];
-    for (int i = tid; i < 6 * 2 * 4 * 3; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
-    for (int i = tid; i < 6 * 2 * 3 * 3; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
+    for (int i = tid; i < 6 * 3 * 2 * 4; i += kBlock) (&S.P[0][0][0][0])[i] = (&tables->P[0][0][0][0])[i];
+    for (int i = tid; i < 6 * 2 * 3 * 4; i += kBlock) (&S.Mb[0][0][0][0])[i] = (&tables->Mb[0][0][0][0])[i];
     for (int i = tid; i < 18; i += kBlock) (&S.tex[0][0])[i] = (&tables->tex[0][0])[i];
     if (tid == 0) S.terrain = -1;
     const bool mesher = ratio > 1 && (int)(blockIdx.x % (uint32_t)ratio) == ratio - 1;
