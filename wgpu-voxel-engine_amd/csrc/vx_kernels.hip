@@ -1401,9 +1401,17 @@ __device__ void emit_rows(const MeshWork &w, uint64_t base, uint32_t F, const Me
                 }
             }
             if (want_i) {
+                // A chunk of one block type needs no load.  Otherwise the id
+                // is used right where it is loaded (clamped to a table row;
+                // ids are at most kBlockDirt, checked at the boundary), so
+                // no load is pending at the loop's back edge, where the
+                // compiler would wait for all of the group's stores.
                 uint32_t blk = w.solid_type;
-                if (valid && blk == kBlockMixed)
-                    blk = w.ids[(size_t)x + (size_t)N * z + (size_t)N * N * y];
+                if (blk == kBlockMixed) {  // uniform
+                    blk = valid ? (uint32_t)w.ids[(size_t)x + (size_t)N * z + (size_t)N * N * y]
+                                : (uint32_t)kBlockGrass;
+                    blk = min(blk, (uint32_t)kBlockDirt);
+                }
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     if (nv <= 32u * h) break;
