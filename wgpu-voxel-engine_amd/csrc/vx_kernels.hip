@@ -365,11 +365,22 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
             const float Z = (float)wzi * fo[o];
             wz[o] = fade(Z - floorf(Z));
         }
+        // The item's cell-change masks in SGPRs: bit r of ychm[o] is set iff
+        // octave o enters a new y cell at row y0 + r (one LDS read per lane
+        // and a ballot per octave, instead of an LDS read per row).
+        uint64_t ychm[OCT];
+        {
+            const uint32_t yc = lane < SEG ? S.ychg[y0 + lane] : 0u;
+#pragma unroll
+            for (int o = 0; o < OCT; o++) ychm[o] = __ballot((yc >> o) & 1u);
+        }
         RowKeep<N> K;
 #pragma unroll 1
         for (int y = y0; y < y0 + SEG; y++) {
-            uint32_t chg = y == y0 ? (1u << OCT) - 1u
-                                   : (uint32_t)__builtin_amdgcn_readfirstlane((int)S.ychg[y]);
+            uint32_t chg = 0;
+#pragma unroll
+            for (int o = 0; o < OCT; o++) chg |= (uint32_t)((ychm[o] >> (y - y0)) & 1ull) << o;
+            if (y == y0) chg = (1u << OCT) - 1u;
             // Rehash the octaves entering a new y cell (rare: a uniform loop
             // over the set bits, so the rehash code exists once).
 #pragma unroll 1
