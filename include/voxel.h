@@ -257,7 +257,9 @@ int vx_frustum_draw_list_noalloc(vx_ctx *ctx, const vx_frustum *f, const int32_t
 
 /* ---- Batches: many Chunk::new + create_mesh jobs in one launch sequence --- */
 /* One request: chunk coord, edge (16, 32 or 64), terrain, and the neighbour
- * snapshot as indices into the same batch (-1 = absent). */
+ * snapshot as indices into the same batch (-1 = absent).  A batch holds 1 to
+ * VX_BATCH_MAX_CHUNKS requests. */
+#define VX_BATCH_MAX_CHUNKS (1 << 26)
 typedef struct vx_batch_req {
     vx_coord coord;
     int32_t edge;

@@ -321,7 +321,8 @@ extern "C" int vx_batch_create(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
 extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_t n,
                                    vx_chunk *const *ext, int32_t n_ext, uint32_t formats,
                                    uint64_t face_capacity, vx_batch **out) {
-    if (!ctx || !out || n <= 0 || !reqs || n_ext < 0 || (n_ext && !ext)) return VX_ERR_INVALID;
+    if (!ctx || !out || n <= 0 || n > VX_BATCH_MAX_CHUNKS || !reqs || n_ext < 0 || (n_ext && !ext))
+        return VX_ERR_INVALID;  // the limit keeps emit item codes (chunk << 2 | slab) in an int
     for (int e = 0; e < n_ext; e++)
         if (!ext[e] || ext[e]->ctx != ctx) return VX_ERR_INVALID;
     if (formats == 0 || (formats & ~(kFmtInstances | kFmtVertices))) return VX_ERR_INVALID;
