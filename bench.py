@@ -517,17 +517,22 @@ def main():
         if m == mode:
             continue
         b.set_mode(m)
-        # a persistent form at the autotuner's fastest grid for it
-        grids = [c["grid"] for c in sorted(tune.get("candidates", []), key=lambda c: c["ms"])
-                 if c["mode"] == names_of[m]]
-        b.set_grid(grids[0] if grids else 0)
+        # a persistent form at the autotuner's fastest grid and roles for it
+        shapes = [(c["grid"], c.get("ratio", 0))
+                  for c in sorted(tune.get("candidates", []), key=lambda c: c["ms"])
+                  if c["mode"] == names_of[m]]
+        b.set_grid(shapes[0][0] if shapes else 0)
+        b.set_roles(shapes[0][1] if shapes else 0)
         b.run()
         assert b.wait() == total_faces
         oe = timed(reps2, 1)
         form_ms[m] = b.profile(args.profile_reps)
         other.append({"form": names_of[m], "value": round(n * reps2 / oe, 1), "unit": "chunks/s",
-                      "ms_per_step": round(oe * 1e3 / reps2, 4)})
+                      "ms_per_step": round(oe * 1e3 / reps2, 4),
+                      "grid": shapes[0][0] if shapes else 0,
+                      "ratio": shapes[0][1] if shapes else 0})
         b.set_grid(0)
+        b.set_roles(0)
     b.set_mode(V.BATCH_AUTO)
     stage_ms = form_ms[V.BATCH_STAGED]
     names = ["fill", "count", "scan", "emit"]

@@ -293,8 +293,9 @@ int vx_batch_destroy(vx_batch *batch);
  *                    neighbour is the adjacent chunk with the same terrain.
  *   VX_BATCH_STAGED  four kernels: fill -> count -> scan -> emit
  *   VX_BATCH_PIPELINED  one kernel with fill and mesh roles: one workgroup in
- *                    three meshes chunks in batch order (count, single-pass
- *                    scan, emit) as their fills and their neighbours' fills
+ *                    three (vx_batch_set_roles) meshes chunks in batch order
+ *                    (count, single-pass scan, emit) as their fills and
+ *                    their neighbours' fills
  *                    publish; the others fill, then mesh once every fill is
  *                    handed out, so noise (VALU) and emission (HBM stores)
  *                    run side by side on every CU; neighbour planes come
@@ -310,6 +311,10 @@ int vx_batch_set_mode(vx_batch *batch, int32_t mode);
  * (0 = one per chunk; fewer = persistent workgroups that keep taking chunk
  * tickets).  The staged form ignores it. */
 int vx_batch_set_grid(vx_batch *batch, int32_t grid);
+/* Roles of the VX_BATCH_PIPELINED launch: one workgroup in `ratio` (2..8)
+ * meshes from the start, the others fill and then mesh; 0 = the default
+ * (one in three).  Other forms ignore it. */
+int vx_batch_set_roles(vx_batch *batch, int32_t ratio);
 /* The form the next run uses (VX_BATCH_FUSED, VX_BATCH_STAGED or
  * VX_BATCH_PIPELINED). */
 int vx_batch_get_mode(vx_batch *batch, int32_t *mode);
@@ -339,11 +344,11 @@ int vx_batch_draw_list(vx_ctx *ctx, vx_batch *batch, const vx_frustum *f, uint32
  * ms[4] = whole run replayed from the graph. */
 int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
 /* Autotuner: times every candidate execution form and launch shape of this
- * batch (staged; fused with one workgroup per chunk; fused with 2..4
+ * batch (staged; fused with one workgroup per chunk and with 3 or 4
  * persistent workgroups per CU; pipelined with one workgroup per chunk and
- * with 4 or 8 workgroups per CU) over `reps` graph replays each, in two
- * rounds (each candidate's faster round counts), keeps the fastest for the
- * batch, and caches the choice for batches of the same shape
+ * one mesher in three, and with 8 workgroups per CU and one mesher in two,
+ * three or four) over `reps` graph replays each, in two rounds (each
+ * candidate's faster round counts), keeps the fastest for the batch, and caches the choice for batches of the same shape
  * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
  * VX_TUNE_CACHE set to a file path the cache persists across processes.
  * Batches created later in VX_BATCH_AUTO use a cached choice. */
@@ -356,6 +361,8 @@ typedef struct vx_tune_result {
     int32_t cand_mode[VX_TUNE_MAX_CANDIDATES];
     int32_t cand_grid[VX_TUNE_MAX_CANDIDATES];
     float cand_ms[VX_TUNE_MAX_CANDIDATES];
+    int32_t ratio; /* chosen pipelined roles (vx_batch_set_roles; 0 elsewhere) */
+    int32_t cand_ratio[VX_TUNE_MAX_CANDIDATES];
 } vx_tune_result;
 int vx_batch_autotune(vx_ctx *ctx, vx_batch *batch, int32_t reps, vx_tune_result *result);
 /* Plain path for comparison: the same work as one launch per chunk per

@@ -222,8 +222,9 @@ def test_generated_chunk_mesh_with_generated_neighbours(ctx, oracle_mod):
 
 # ------------------------------------------------------------------ batches
 def apply_mode(b, mode):
-    """mode: a VX_BATCH_* value or (mode, grid), grid = persistent workgroups."""
-    m, g = mode if isinstance(mode, tuple) else (mode, 0)
+    """mode: a VX_BATCH_* value or (mode, grid[, ratio]), grid = persistent
+    workgroups, ratio = one pipelined mesher workgroup in `ratio`."""
+    m, g, r = (tuple(mode) + (0,))[:3] if isinstance(mode, tuple) else (mode, 0, 0)
     if m == V.BATCH_PIPELINED and int(np.max(b.edges)) > 32:
         with pytest.raises(V.VoxelError) as e:
             b.set_mode(m)
@@ -231,6 +232,7 @@ def apply_mode(b, mode):
         pytest.skip("the pipelined form takes edges up to 32 (refused as specified)")
     b.set_mode(m)
     b.set_grid(g)
+    b.set_roles(r)
 
 
 def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES,
@@ -238,6 +240,11 @@ def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES 
     tids = [ctx.terrain(t) for t in terrains]
     b = V.Batch(ctx, coords, edges, tids, nb, formats=formats, face_capacity=face_capacity)
     apply_mode(b, mode)
+    return check_batch(b, O, coords, edges, terrains, nb, formats, per_chunk)
+
+
+def check_batch(b, O, coords, edges, terrains, nb, formats, per_chunk=False):
+    """Runs batch `b` and checks its ids and meshes against the oracle."""
     if per_chunk:
         b.run_per_chunk()
     else:
@@ -267,13 +274,15 @@ def batch_vs_oracle(ctx, O, coords, edges, terrains, nb, formats=V.FMT_VERTICES 
 
 # fused with one workgroup per chunk; fused with 3 persistent workgroups that
 # take every chunk ticket in turn; staged; pipelined with one workgroup per
-# chunk, with 3 workgroups (two fill, one meshes from the start), and with a
-# single workgroup (a filler: it fills every chunk, then meshes them all)
+# chunk, with 3 workgroups (two fill, one meshes from the start), with 4
+# workgroups of which two mesh from the start, and with a single workgroup
+# (a filler: it fills every chunk, then meshes them all)
 MODES = [pytest.param((V.BATCH_FUSED, 0), id="fused"),
          pytest.param((V.BATCH_FUSED, 3), id="fused_persistent3"),
          pytest.param((V.BATCH_STAGED, 0), id="staged"),
          pytest.param((V.BATCH_PIPELINED, 0), id="pipelined"),
          pytest.param((V.BATCH_PIPELINED, 3), id="pipelined_persistent3"),
+         pytest.param((V.BATCH_PIPELINED, 4, 2), id="pipelined_persistent4_roles2"),
          pytest.param((V.BATCH_PIPELINED, 1), id="pipelined_single")]
 
 
@@ -289,13 +298,40 @@ def test_batch_config1_grid(ctx, oracle_mod, mode):
 @pytest.mark.parametrize("mode,per_chunk", [(V.BATCH_FUSED, False), (V.BATCH_STAGED, False),
                                             (V.BATCH_STAGED, True), ((V.BATCH_FUSED, 1), False),
                                             (V.BATCH_PIPELINED, False),
-                                            ((V.BATCH_PIPELINED, 7), False)])
+                                            ((V.BATCH_PIPELINED, 7), False),
+                                            ((V.BATCH_PIPELINED, 9, 4), False),
+                                            ((V.BATCH_PIPELINED, 16, 8), False)])
 def test_batch_config2_subgrid(ctx, oracle_mod, mode, per_chunk):
     """Config 2 terrain on an 8x8 sub-grid of 32^3 chunks."""
     say("batch C2 8x8", mode, per_chunk)
     coords = [(x + 20, 0, z + 30) for x in range(8) for z in range(8)]
     batch_vs_oracle(ctx, oracle_mod, coords, [32] * 64, [C2] * 64, V.grid_neighbours(8, 8),
                     per_chunk=per_chunk, mode=mode)
+
+
+def test_batch_autotune_keeps_results(ctx, oracle_mod):
+    """The autotuner times the staged, fused and pipelined forms (the last
+    with one mesher workgroup in three) and keeps the fastest; the tuned
+    batch's ids and meshes are the oracle's, byte for byte.  Role splits
+    outside 2..8 are refused."""
+    say("batch autotune")
+    coords = [(x + 20, 0, z + 30) for x in range(8) for z in range(8)]
+    nb = V.grid_neighbours(8, 8)
+    fmt = V.FMT_VERTICES | V.FMT_INSTANCES
+    b = V.Batch(ctx, coords, [32] * 64, [ctx.terrain(C2)] * 64, nb, formats=fmt)
+    r = b.autotune(reps=2)
+    kinds = {(c["mode"], c["ratio"]) for c in r["candidates"]}
+    assert {("staged", 0), ("fused", 0), ("pipelined", 3)} <= kinds, r
+    assert r["mode"] in ("staged", "fused", "pipelined")
+    chosen = [c for c in r["candidates"]
+              if (c["mode"], c["grid"], c["ratio"]) == (r["mode"], r["grid"], r["ratio"])]
+    assert chosen and chosen[0]["ms"] == min(c["ms"] for c in r["candidates"]), r
+    for bad in (1, 9, -1):
+        with pytest.raises(V.VoxelError) as e:
+            b.set_roles(bad)
+        assert e.value.status == V.ERR_INVALID
+    check_batch(b, oracle_mod, coords, [32] * 64, [C2] * 64, nb, fmt)
+    b.close()
 
 
 @pytest.mark.parametrize("mode", MODES)

@@ -186,13 +186,17 @@ int32_t fused_grid_of(const vx_batch *b) {
     return b->mode == VX_BATCH_AUTO && b->tuned ? b->tuned_grid : b->grid;
 }
 
+int32_t pipe_ratio_of(const vx_batch *b) {
+    return b->mode == VX_BATCH_AUTO && b->tuned ? b->tuned_ratio : b->pipe_ratio;
+}
+
 hipError_t enqueue_fused(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
     if (form_of(b) == VX_BATCH_PIPELINED)
         return launch_pipe(b->fused_dev, b->n, b->max_edge, b->pipe_lag, ctx->terrains_dev,
                            ctx->lut_dev,
                            ctx->tables_dev, b->fctl_ptrs, b->out, b->cull, fused_grid_of(b),
-                           ctx->stream);
+                           pipe_ratio_of(b), ctx->stream);
     return launch_fused(b->fused_dev, b->n, b->max_edge, ctx->terrains_dev, ctx->lut_dev,
                         ctx->tables_dev, b->fctl_ptrs, b->out, b->cull, fused_grid_of(b),
                         ctx->stream);
@@ -265,7 +269,7 @@ int fetch_results(vx_batch *b) {
 // largest edge, uniform or ragged, fused form eligible, has neighbours.
 using TuneKey = std::tuple<int, int, int, int, int>;
 struct TuneChoice {
-    int32_t mode, grid;
+    int32_t mode, grid, ratio;  // ratio: pipelined roles (0 = default)
 };
 std::map<TuneKey, TuneChoice> g_tune;
 std::mutex g_tune_mu;
@@ -281,7 +285,8 @@ TuneKey tune_key(const vx_batch *b) {
 }
 
 // The cache persists across processes when VX_TUNE_CACHE names a file:
-// one line per shape, "bucket max_edge uniform fused_ok has_nb mode grid".
+// one line per shape, "bucket max_edge uniform fused_ok has_nb mode grid
+// ratio".
 void tune_load() {
     if (g_tune_loaded) return;
     g_tune_loaded = true;
@@ -289,9 +294,9 @@ void tune_load() {
     if (!path) return;
     FILE *f = fopen(path, "r");
     if (!f) return;
-    int a, c, d, e, h, m, g;
-    while (fscanf(f, "%d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g) == 7)
-        g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g};
+    int a, c, d, e, h, m, g, r;
+    while (fscanf(f, "%d %d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g, &r) == 8)
+        if (r == 0 || (r >= 2 && r <= kPipeRatioMax)) g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g, r};
     fclose(f);
 }
 
@@ -301,9 +306,9 @@ void tune_save() {
     FILE *f = fopen(path, "w");
     if (!f) return;
     for (auto &kv : g_tune)
-        fprintf(f, "%d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
+        fprintf(f, "%d %d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
                 std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first),
-                kv.second.mode, kv.second.grid);
+                kv.second.mode, kv.second.grid, kv.second.ratio);
     fclose(f);
 }
 
@@ -547,6 +552,7 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
             b->tuned = true;
             b->tuned_mode = it->second.mode;
             b->tuned_grid = it->second.grid;
+            b->tuned_ratio = it->second.ratio;
         }
     }
     *out = b;
@@ -581,6 +587,17 @@ extern "C" int vx_batch_set_grid(vx_batch *b, int32_t grid) {
         (void)hipStreamSynchronize(b->ctx->stream);
         drop_graph(b);
         b->grid = grid;
+    }
+    return VX_OK;
+}
+
+extern "C" int vx_batch_set_roles(vx_batch *b, int32_t ratio) {
+    if (!b || (ratio != 0 && (ratio < 2 || ratio > kPipeRatioMax))) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    if (ratio != b->pipe_ratio) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        drop_graph(b);
+        b->pipe_ratio = ratio;
     }
     return VX_OK;
 }
@@ -839,16 +856,18 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
     int cus = 1;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     std::vector<TuneChoice> cands;
-    cands.push_back({VX_BATCH_STAGED, 0});
+    cands.push_back({VX_BATCH_STAGED, 0, 0});
     if (b->fused_ok) {
-        cands.push_back({VX_BATCH_FUSED, 0});
-        for (int per = 2; per <= 4; per++)
-            if (cus * per < b->n) cands.push_back({VX_BATCH_FUSED, cus * per});
+        cands.push_back({VX_BATCH_FUSED, 0, 0});
+        for (int per = 3; per <= 4; per++)
+            if (cus * per < b->n) cands.push_back({VX_BATCH_FUSED, cus * per, 0});
     }
-    if (b->pipe_ok) {  // fill and mesh roles: one workgroup per chunk, or 4 or 8 per CU
-        cands.push_back({VX_BATCH_PIPELINED, 0});
-        for (int per = 4; per <= 8; per += 4)
-            if (cus * per < b->n) cands.push_back({VX_BATCH_PIPELINED, cus * per});
+    if (b->pipe_ok) {
+        // fill and mesh roles: one workgroup per chunk with one mesher in
+        // three; 8 per CU with one mesher in two, three or four
+        cands.push_back({VX_BATCH_PIPELINED, 0, 3});
+        for (int ratio = 2; ratio <= 4; ratio++)
+            if (cus * 8 < b->n) cands.push_back({VX_BATCH_PIPELINED, cus * 8, ratio});
     }
     VX_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a queued replay may still run
     hipEvent_t e0, e1;
@@ -866,6 +885,7 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
             drop_graph(b);
             b->mode = cands[c].mode;
             b->grid = cands[c].grid;
+            b->pipe_ratio = cands[c].ratio;
             if ((st = capture_graph(b))) break;
             hipError_t e = hipGraphLaunch(b->graph_exec, ctx->stream);  // warm-up
             if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
@@ -887,6 +907,7 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
         if (c < VX_TUNE_MAX_CANDIDATES) {
             r.cand_mode[c] = cands[c].mode;
             r.cand_grid[c] = cands[c].grid;
+            r.cand_ratio[c] = cands[c].ratio;
             r.cand_ms[c] = tmin[c];
             r.n_candidates = (int32_t)c + 1;
         }
@@ -897,14 +918,17 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
     drop_graph(b);
     b->mode = VX_BATCH_AUTO;
     b->grid = 0;
+    b->pipe_ratio = 0;
     b->tuned = true;
     b->tuned_mode = bestc.mode;
     b->tuned_grid = bestc.grid;
+    b->tuned_ratio = bestc.ratio;
     b->ran = true;
     b->results = false;
     if (st) return st;
     r.mode = bestc.mode;
     r.grid = bestc.grid;
+    r.ratio = bestc.ratio;
     r.best_ms = best;
     {
         std::lock_guard<std::mutex> tg(g_tune_mu);

@@ -197,13 +197,16 @@ hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
                         const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
                         FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
                         hipStream_t s);
-// Pipelined persistent form (edges up to 32): fill and mesh tasks from two
-// ticket counters; mesh m is handed out once fills up to m + lag are out
-// (lag >= the farthest in-batch neighbour index minus the chunk's own).
-// Needs ctl.filled and ctl.mesh_ticket.
+// Pipelined form (edges up to 32): fill and mesh tasks from two ticket
+// counters; one workgroup in `ratio` (2..kPipeRatioMax, 0 = kPipeRatio)
+// meshes from the start, the others fill and then mesh; mesh m is served
+// once fills up to m + lag are out (lag >= the farthest in-batch neighbour
+// index minus the chunk's own).  Needs ctl.filled and ctl.mesh_ticket.
+constexpr int kPipeRatio = 3;
+constexpr int kPipeRatioMax = 8;
 hipError_t launch_pipe(const FusedWork *work, int32_t n, int32_t max_edge, int32_t lag,
                        const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
-                       FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
+                       FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid, int32_t ratio,
                        hipStream_t s);
 hipError_t launch_cull(const CullJob &job, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s);

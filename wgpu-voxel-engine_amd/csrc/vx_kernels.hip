@@ -1918,8 +1918,6 @@ __device__ void pipe_mesh(const FusedWork &W, int m, int n, const FusedCtl &ctl,
     __syncthreads();
 }
 
-// One workgroup in kPipeRatio of the pipelined grid meshes from the start.
-constexpr int kPipeRatio = 3;
 
 // Register cap: four waves per SIMD, the fill kernel's budget.  Roles:
 // workgroup b meshes from the start when b % ratio == ratio - 1; the others
@@ -2225,17 +2223,19 @@ hipError_t launch_copy(void *dst, const void *src, uint64_t bytes, hipStream_t s
 
 hipError_t launch_pipe(const FusedWork *work, int32_t n, int32_t max_edge, int32_t lag,
                        const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
-                       FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,
+                       FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid, int32_t ratio,
                        hipStream_t s) {
     if (n <= 0) return hipSuccess;
     if (max_edge > 32 || !ctl.filled || !ctl.mesh_ticket || lag < 0) return hipErrorInvalidValue;
+    if (ratio == 0) ratio = kPipeRatio;
+    if (ratio < 2 || ratio > kPipeRatioMax) return hipErrorInvalidValue;
     fused_reset_kernel<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(ctl, n);
     if (max_edge <= 16)
         pipe_kernel<16><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, lag, terrains, lut, tables,
-                                                               ctl, out, cull, kPipeRatio);
+                                                               ctl, out, cull, ratio);
     else
         pipe_kernel<32><<<fused_grid(n, grid), kBlock, 0, s>>>(work, n, lag, terrains, lut, tables,
-                                                               ctl, out, cull, kPipeRatio);
+                                                               ctl, out, cull, ratio);
     return hipGetLastError();
 }
 

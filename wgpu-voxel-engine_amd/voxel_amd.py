@@ -108,7 +108,8 @@ class SimResult(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("mode", C.c_int32), ("grid", C.c_int32), ("best_ms", C.c_float),
                 ("n_candidates", C.c_int32), ("cand_mode", C.c_int32 * 8),
-                ("cand_grid", C.c_int32 * 8), ("cand_ms", C.c_float * 8)]
+                ("cand_grid", C.c_int32 * 8), ("cand_ms", C.c_float * 8),
+                ("ratio", C.c_int32), ("cand_ratio", C.c_int32 * 8)]
 
 
 class WorldDesc(C.Structure):
@@ -182,6 +183,7 @@ _SIGS = {
     "vx_batch_run": (C.c_int, [P, P]),
     "vx_batch_set_mode": (C.c_int, [P, C.c_int32]),
     "vx_batch_set_grid": (C.c_int, [P, C.c_int32]),
+    "vx_batch_set_roles": (C.c_int, [P, C.c_int32]),
     "vx_batch_get_mode": (C.c_int, [P, C.POINTER(C.c_int32)]),
     "vx_batch_run_per_chunk": (C.c_int, [P, P]),
     "vx_batch_wait": (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
@@ -587,6 +589,10 @@ class Batch:
         """Workgroups of the fused launch (0 = one per chunk); staged ignores it."""
         self.ctx.check(self.ctx.L.vx_batch_set_grid(self.h, grid), "vx_batch_set_grid")
 
+    def set_roles(self, ratio: int):
+        """Pipelined form: one mesher workgroup in `ratio` (2..8; 0 = default)."""
+        self.ctx.check(self.ctx.L.vx_batch_set_roles(self.h, ratio), "vx_batch_set_roles")
+
     @property
     def mode(self) -> int:
         m = C.c_int32()
@@ -636,9 +642,11 @@ class Batch:
         self.ctx.check(self.ctx.L.vx_batch_autotune(self.ctx.h, self.h, reps, C.byref(r)),
                        "vx_batch_autotune")
         names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged", BATCH_PIPELINED: "pipelined"}
-        return {"mode": names.get(r.mode, r.mode), "grid": r.grid, "best_ms": round(r.best_ms, 4),
+        return {"mode": names.get(r.mode, r.mode), "grid": r.grid, "ratio": r.ratio,
+                "best_ms": round(r.best_ms, 4),
                 "candidates": [{"mode": names.get(r.cand_mode[i], r.cand_mode[i]),
-                                "grid": r.cand_grid[i], "ms": round(r.cand_ms[i], 4)}
+                                "grid": r.cand_grid[i], "ratio": r.cand_ratio[i],
+                                "ms": round(r.cand_ms[i], 4)}
                                for i in range(r.n_candidates)]}
 
     def profile(self, reps: int = 10) -> np.ndarray:
