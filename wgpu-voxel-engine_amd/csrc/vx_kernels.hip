@@ -104,6 +104,7 @@ template <int MAXN> struct RowStride { static constexpr int v = MAXN == 64 ? 2 :
 // LDS layout of one fill workgroup, sized for the largest edge it handles.
 template <int MAXN> struct FillShared {
     int perm[512];
+    int plut[512];                     // (perm[i] & 15) * 16: byte offset of the gradient in a lut row
     float4 lut[8][16];
     float4 ytab[kMaxOctaves * MAXN];   // per (octave, y): fy, fy-1, fade(fy), Y cell
     uint32_t ychg[MAXN];               // per y: bit o set iff octave o's Y cell differs from y-1
@@ -200,6 +201,26 @@ __device__ __forceinline__ void corner_hashes(const SH &S, int px0, int px1, int
         const int BA = S.perm[B] + yrow, BB = S.perm[B + 1] + yrow;
         h[0] = S.perm[AA]; h[1] = S.perm[BA]; h[2] = S.perm[AA + 1]; h[3] = S.perm[BA + 1];
         h[4] = S.perm[AB]; h[5] = S.perm[BB]; h[6] = S.perm[AB + 1]; h[7] = S.perm[BB + 1];
+    }
+}
+
+// corner_hashes' last level read from S.plut: the byte offset of each
+// corner's gradient within its lut row, (hash & 15) * 16.
+template <bool REF, class SH>
+__device__ __forceinline__ void corner_offsets(const SH &S, int px0, int px1, int yrow, int zcol,
+                                               int h[8]) {
+    if (!REF) {
+        const int A = px0 + yrow, B = px1 + yrow;
+        const int AA = S.perm[A] + zcol, AB = S.perm[A + 1] + zcol;
+        const int BA = S.perm[B] + zcol, BB = S.perm[B + 1] + zcol;
+        h[0] = S.plut[AA]; h[1] = S.plut[BA]; h[2] = S.plut[AB]; h[3] = S.plut[BB];
+        h[4] = S.plut[AA + 1]; h[5] = S.plut[BA + 1]; h[6] = S.plut[AB + 1]; h[7] = S.plut[BB + 1];
+    } else {
+        const int A = px0 + zcol, B = px1 + zcol;
+        const int AA = S.perm[A] + yrow, AB = S.perm[A + 1] + yrow;
+        const int BA = S.perm[B] + yrow, BB = S.perm[B + 1] + yrow;
+        h[0] = S.plut[AA]; h[1] = S.plut[BA]; h[2] = S.plut[AA + 1]; h[3] = S.plut[BA + 1];
+        h[4] = S.plut[AB]; h[5] = S.plut[BB]; h[6] = S.plut[AB + 1]; h[7] = S.plut[BB + 1];
     }
 }
 
@@ -398,11 +419,12 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
                 const int Xi = cell8(fxl), zi = cell8(fzl);
                 const int px0 = S.perm[Xi], px1 = S.perm[Xi + 1];
                 int h[8];
-                corner_hashes<REF>(S, px0, px1, Yi, zi, h);
+                corner_offsets<REF>(S, px0, px1, Yi, zi, h);
                 float cn[8], gn[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const float4 Lk = S.lut[k][h[k] & 15];
+                    const float4 Lk = *reinterpret_cast<const float4 *>(
+                        reinterpret_cast<const char *>(&S.lut[k][0]) + h[k]);
                     const float xr = (k & 1) ? fx1 : fx;
                     const float zr = (k & 4) ? fz1 : fz;
                     cn[k] = fmaf(Lk.x, xr, Lk.y * zr);
@@ -413,7 +435,9 @@ __device__ void density_lines_oct(const TerrainDev &T, int ox, int oy, int oz, c
                 // coefficient of every octave costs a v_cndmask per rehash
                 // (64 at four octaves, each reading its mask from SGPRs at
                 // half issue rate); the asm keeps the branches from being
-                // turned back into selects.
+                // turned back into selects.  (One copy of the rehash per
+                // octave, writing its registers in place, raised the
+                // register count past four waves per SIMD and spilled.)
 #pragma unroll
                 for (int q = 0; q < OCT; q++) {
                     if (o == q) {
@@ -526,15 +550,24 @@ template <int N, int MAXN, bool DBG, int SEG, bool OCT_INNER, class SH>
 __device__ void generate_rows(const TerrainDev &T, int ox, int oy, int oz, const Halo &H, SH &S,
                               float *noise_out) {
     using R = typename RowType<MAXN>::T;
-    if (T.kind == kDensity && !OCT_INNER) {
-        build_ytab<N>(T, oy, S);
-        __syncthreads();
-        density_lines<N, MAXN, DBG, SEG, false>(T, ox, oy, oz, H, S, noise_out);
-    } else if (T.kind == kDensity) {
-        density_rows<N, MAXN, DBG, SEG, false>(T, ox, oy, oz, H, S, noise_out);
-    } else if (OCT_INNER && T.kind == kReference) {
-        density_rows<N, MAXN, DBG, SEG, true>(T, ox, oy, oz, H, S, noise_out);
-    } else if (T.kind == kHeightmap) {
+    if constexpr (!OCT_INNER) {
+        if (T.kind == kDensity) {
+            build_ytab<N>(T, oy, S);
+            __syncthreads();
+            density_lines<N, MAXN, DBG, SEG, false>(T, ox, oy, oz, H, S, noise_out);
+            return;
+        }
+    } else {
+        if (T.kind == kDensity) {
+            density_rows<N, MAXN, DBG, SEG, false>(T, ox, oy, oz, H, S, noise_out);
+            return;
+        }
+        if (T.kind == kReference) {
+            density_rows<N, MAXN, DBG, SEG, true>(T, ox, oy, oz, H, S, noise_out);
+            return;
+        }
+    }
+    if (T.kind == kHeightmap) {
         heightmap_lines<N, MAXN, DBG>(T, ox, oy, oz, H, S, noise_out);
     } else {
         const R full = T.kind == kSolid ? (N == 64 ? (R)~(R)0 : (R)((N >= 32 ? 0xFFFFFFFFu
@@ -741,7 +774,11 @@ __device__ __forceinline__ void fill_item(const FillWork &w, const TerrainDev *t
                                           int parts = 1) {
     const TerrainDev &T = terrains[w.terrain];
     const float4(*L)[16] = T.kind == kReference ? lut->r : lut->c;
-    for (int i = threadIdx.x; i < 512; i += kBlock) S.perm[i] = T.perm[i];
+    for (int i = threadIdx.x; i < 512; i += kBlock) {
+        const int p = T.perm[i];
+        S.perm[i] = p;
+        S.plut[i] = (p & 15) * 16;
+    }
     for (int i = threadIdx.x; i < 8 * 16; i += kBlock) S.lut[i >> 4][i & 15] = L[i >> 4][i & 15];
     __syncthreads();
     if (w.noise_out) {  // parity-check path: also stores the raw noise
@@ -1821,6 +1858,7 @@ __global__ void __launch_bounds__(kBlock) fused_reset_kernel(FusedCtl ctl, int32
 // flag is published.
 template <int MAXN> struct PipeShared {
     int perm[512];
+    int plut[512];
     float4 lut[8][16];
     float4 ytab[kMaxOctaves * MAXN];
     uint32_t ychg[MAXN];
@@ -1951,7 +1989,11 @@ pipe_kernel(const FusedWork *__restrict__ work, int32_t n, int32_t lag,
             const FusedWork &W = work[t];
             const int terrain = W.f.terrain;
             if (terrain != S.terrain) {
-                for (int i = tid; i < 512; i += kBlock) S.perm[i] = terrains[terrain].perm[i];
+                for (int i = tid; i < 512; i += kBlock) {
+                    const int p = terrains[terrain].perm[i];
+                    S.perm[i] = p;
+                    S.plut[i] = (p & 15) * 16;
+                }
                 __syncthreads();
                 if (tid == 0) S.terrain = terrain;
             }
