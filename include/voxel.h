@@ -309,11 +309,13 @@ int vx_batch_destroy(vx_batch *batch);
 int vx_batch_set_mode(vx_batch *batch, int32_t mode);
 /* Workgroups of the persistent launch in VX_BATCH_FUSED / VX_BATCH_PIPELINED
  * (0 = one per chunk; fewer = persistent workgroups that keep taking chunk
- * tickets).  The staged form ignores it. */
+ * tickets).  The staged form ignores it, and so does VX_BATCH_AUTO once the
+ * autotuner has chosen a shape for the batch. */
 int vx_batch_set_grid(vx_batch *batch, int32_t grid);
 /* Roles of the VX_BATCH_PIPELINED launch: one workgroup in `ratio` (2..8)
  * meshes from the start, the others fill and then mesh; 0 = the default
- * (one in three).  Other forms ignore it. */
+ * (one in three).  Other forms ignore it, and so does VX_BATCH_AUTO once the
+ * autotuner has chosen a split for the batch. */
 int vx_batch_set_roles(vx_batch *batch, int32_t ratio);
 /* The form the next run uses (VX_BATCH_FUSED, VX_BATCH_STAGED or
  * VX_BATCH_PIPELINED). */
