@@ -294,9 +294,17 @@ void tune_load() {
     if (!path) return;
     FILE *f = fopen(path, "r");
     if (!f) return;
-    int a, c, d, e, h, m, g, r;
-    while (fscanf(f, "%d %d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g, &r) == 8)
-        if (r == 0 || (r >= 2 && r <= kPipeRatioMax)) g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g, r};
+    // line by line: a line of the earlier seven-field form (no ratio) reads
+    // as the default split; a line that is neither form is skipped
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        int a, c, d, e, h, m, g, r = 0;
+        const int k = sscanf(line, "%d %d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g, &r);
+        if (k != 7 && k != 8) continue;
+        const bool mode_ok = m == VX_BATCH_STAGED || m == VX_BATCH_FUSED || m == VX_BATCH_PIPELINED;
+        const bool ratio_ok = r == 0 || (r >= 2 && r <= kPipeRatioMax);
+        if (mode_ok && ratio_ok && g >= 0) g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g, r};
+    }
     fclose(f);
 }
 
