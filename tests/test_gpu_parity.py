@@ -421,6 +421,31 @@ def test_coordinates_outside_the_i32_world_are_refused(ctx, oracle_mod):
         assert (ch.ids() == oracle_mod.chunk_fill(oterrain(oracle_mod, BIGF), *coord, 32)).all()
 
 
+def test_batch_sizes_outside_the_range_are_refused(ctx):
+    """A batch holds 1 to VX_BATCH_MAX_CHUNKS (2^26) requests: an empty batch
+    and a count one above the maximum are refused with VX_ERR_INVALID (the
+    count is checked before any request is read), and a one-chunk batch
+    runs."""
+    import ctypes as C
+    tid = ctx.terrain(C1)
+    with pytest.raises(V.VoxelError) as e:
+        V.Batch(ctx, np.zeros((0, 3), np.int32), 16, tid)
+    assert e.value.status == V.ERR_INVALID
+    one = (V.BatchReq * 1)()
+    one[0].edge = 16
+    one[0].terrain_id = tid
+    for k in range(4):
+        one[0].nb[k] = -1
+    out = C.c_void_p()
+    st = ctx.L.vx_batch_create(ctx.h, C.addressof(one), (1 << 26) + 1, V.FMT_VERTICES, 0,
+                               C.byref(out))
+    assert st == V.ERR_INVALID and not out.value
+    b = V.Batch(ctx, [(0, 0, 0)], 16, tid)
+    b.run()
+    assert b.wait() > 0
+    b.close()
+
+
 def test_terrain_validation(ctx):
     """Octave counts outside 1..8 and octave frequencies below 2^-20 are
     refused with VX_ERR_INVALID."""
