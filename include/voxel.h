@@ -317,6 +317,14 @@ int vx_batch_set_grid(vx_batch *batch, int32_t grid);
  * (one in three).  Other forms ignore it, and so does VX_BATCH_AUTO once the
  * autotuner has chosen a split for the batch. */
 int vx_batch_set_roles(vx_batch *batch, int32_t ratio);
+/* Tile shape of the VX_BATCH_STAGED form for small batches: workgroups per
+ * chunk of the fill (1, 2, 4 or 8; used below 1,024 chunks, where the rule
+ * is 2, 4 or 8, and 8 only for 64^3 chunks) and per item of the emit (1..4;
+ * used below 1,280 items, where the rule is 2, 3 or 4); 0 = the rule.
+ * Larger batches take one workgroup per chunk and per item whatever is set.
+ * Other forms ignore it, and so does VX_BATCH_AUTO once the autotuner has
+ * chosen a shape for the batch. */
+int vx_batch_set_splits(vx_batch *batch, int32_t fill_parts, int32_t emit_parts);
 /* The form the next run uses (VX_BATCH_FUSED, VX_BATCH_STAGED or
  * VX_BATCH_PIPELINED). */
 int vx_batch_get_mode(vx_batch *batch, int32_t *mode);
@@ -346,11 +354,14 @@ int vx_batch_draw_list(vx_ctx *ctx, vx_batch *batch, const vx_frustum *f, uint32
  * ms[4] = whole run replayed from the graph. */
 int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
 /* Autotuner: times every candidate execution form and launch shape of this
- * batch (staged; fused with one workgroup per chunk and with 3 or 4
- * persistent workgroups per CU; pipelined with one workgroup per chunk and
- * one mesher in three, and with 8 workgroups per CU and one mesher in two,
- * three or four) over `reps` graph replays each, in two rounds (each
- * candidate's faster round counts), keeps the fastest for the batch, and caches the choice for batches of the same shape
+ * batch (staged; for small batches also staged with other tile shapes,
+ * vx_batch_set_splits: half and twice the fill's workgroups per chunk, one
+ * and four (or two) emit workgroups per item; fused with one workgroup per
+ * chunk and with 3 or 4 persistent workgroups per CU; pipelined with one
+ * workgroup per chunk and one mesher in three, and with 8 workgroups per CU
+ * and one mesher in two, three or four) over `reps` graph replays each, in
+ * two rounds (each candidate's faster round counts), keeps the fastest for
+ * the batch, and caches the choice for batches of the same shape
  * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
  * VX_TUNE_CACHE set to a file path the cache persists across processes.
  * Batches created later in VX_BATCH_AUTO use a cached choice. */
@@ -365,6 +376,10 @@ typedef struct vx_tune_result {
     float cand_ms[VX_TUNE_MAX_CANDIDATES];
     int32_t ratio; /* chosen pipelined roles (vx_batch_set_roles; 0 elsewhere) */
     int32_t cand_ratio[VX_TUNE_MAX_CANDIDATES];
+    int32_t fill_split; /* chosen staged tile shape (vx_batch_set_splits; 0 = the rule) */
+    int32_t emit_split;
+    int32_t cand_fill_split[VX_TUNE_MAX_CANDIDATES];
+    int32_t cand_emit_split[VX_TUNE_MAX_CANDIDATES];
 } vx_tune_result;
 int vx_batch_autotune(vx_ctx *ctx, vx_batch *batch, int32_t reps, vx_tune_result *result);
 /* Plain path for comparison: the same work as one launch per chunk per

@@ -322,15 +322,43 @@ def test_batch_autotune_keeps_results(ctx, oracle_mod):
     r = b.autotune(reps=2)
     kinds = {(c["mode"], c["ratio"]) for c in r["candidates"]}
     assert {("staged", 0), ("fused", 0), ("pipelined", 3)} <= kinds, r
+    # 64 chunks: the fill and the emit split chunks, so staged tile shapes
+    # other than the rule are among the candidates
+    shapes = {tuple(c["splits"]) for c in r["candidates"] if c["mode"] == "staged"}
+    assert (0, 0) in shapes and any(f for f, _ in shapes) and any(e for _, e in shapes), r
     assert r["mode"] in ("staged", "fused", "pipelined")
     chosen = [c for c in r["candidates"]
-              if (c["mode"], c["grid"], c["ratio"]) == (r["mode"], r["grid"], r["ratio"])]
+              if (c["mode"], c["grid"], c["ratio"], c["splits"]) ==
+              (r["mode"], r["grid"], r["ratio"], r["splits"])]
     assert chosen and chosen[0]["ms"] == min(c["ms"] for c in r["candidates"]), r
     for bad in (1, 9, -1):
         with pytest.raises(V.VoxelError) as e:
             b.set_roles(bad)
         assert e.value.status == V.ERR_INVALID
     check_batch(b, oracle_mod, coords, [32] * 64, [C2] * 64, nb, fmt)
+    b.close()
+
+
+@pytest.mark.parametrize("splits", [(1, 1), (2, 4), (4, 2), (8, 3), (0, 1), (1, 0)])
+@pytest.mark.parametrize("edge", [32, 64])
+def test_batch_staged_tile_shapes(ctx, oracle_mod, splits, edge):
+    """The staged form at set tile shapes (vx_batch_set_splits): fill
+    workgroups per chunk splitting the density rows, emit workgroups per
+    item splitting its tiles, on a 3x3 grid with in-grid neighbours; ids and
+    meshes byte for byte (at edge 32 a fill split of 8 uses 4 of them).
+    Values outside 1, 2, 4, 8 and 1..4 are refused."""
+    say("batch tile shapes", splits, edge)
+    coords = [(x - 1, 0, z + 2) for x in range(3) for z in range(3)]
+    nb = V.grid_neighbours(3, 3)
+    fmt = V.FMT_VERTICES | V.FMT_INSTANCES
+    b = V.Batch(ctx, coords, edge, ctx.terrain(C2), nb, formats=fmt)
+    b.set_mode(V.BATCH_STAGED)
+    for bad in ((3, 0), (16, 0), (-1, 0), (0, 5), (0, -1)):
+        with pytest.raises(V.VoxelError) as e:
+            b.set_splits(*bad)
+        assert e.value.status == V.ERR_INVALID
+    b.set_splits(*splits)
+    check_batch(b, oracle_mod, coords, [edge] * 9, [C2] * 9, nb, fmt)
     b.close()
 
 

@@ -1,8 +1,9 @@
 """The autotuner's cache file (VX_TUNE_CACHE): vx_batch_autotune writes one
 line per batch shape, "bucket max_edge uniform fused_ok has_nb mode grid
-ratio"; the first batch a process creates reads the file once.  Each step
-below runs in a fresh Python process so that the file is read again; every
-run checks that the form the cache selects produces the same face count.
+ratio fill_split emit_split"; the first batch a process creates reads the
+file once.  Each step below runs in a fresh Python process so that the file
+is read again; every run checks that the form the cache selects produces
+the same face count.
 """
 import os
 import subprocess
@@ -51,10 +52,11 @@ def test_tune_cache_round_trip_and_old_lines(tmp_path):
     cache = str(tmp_path / "tune.txt")
     mode, faces = run("tune", cache)
     lines = [x.split() for x in open(cache).read().splitlines() if x.strip()]
-    assert len(lines) == 1 and len(lines[0]) == 8, lines
+    assert len(lines) == 1 and len(lines[0]) == 10, lines
     key = lines[0][:5]
     assert int(lines[0][5]) == mode
     assert int(lines[0][7]) in (0, 2, 3, 4)
+    assert int(lines[0][8]) in (0, 1, 2, 4, 8) and int(lines[0][9]) in range(5)
 
     def write(*rows):
         with open(cache, "w") as f:
@@ -70,6 +72,13 @@ def test_tune_cache_round_trip_and_old_lines(tmp_path):
     # the earlier seven-field form still reads (default role split)
     write(key + [V.BATCH_FUSED, 0])
     assert run("run", cache) == (V.BATCH_FUSED, faces)
+    # the staged form at a tile shape (fill 1 and emit 4 workgroups)
+    write(key + [V.BATCH_STAGED, 0, 0, 1, 4])
+    assert run("run", cache) == (V.BATCH_STAGED, faces)
+    # a tile shape no setter accepts is not loaded: the batch takes the rule
+    # (staged, as with no entry), shown with a pipelined entry
+    write(key + [V.BATCH_PIPELINED, 0, 3, 3, 0])
+    assert run("run", cache) == (V.BATCH_STAGED, faces)
     # a role split outside 2..8 is not loaded: the batch runs staged
     write(key + [V.BATCH_PIPELINED, 0, 1])
     assert run("run", cache) == (V.BATCH_STAGED, faces)

@@ -145,10 +145,21 @@ bool schedules_ok(vx_batch *b) {
     return is_item_schedule(h.data(), b->n_items, per.data(), b->n);
 }
 
+// The staged form's workgroups per chunk (fill) and per item (emit): the
+// autotuner's choice under VX_BATCH_AUTO, else vx_batch_set_splits' (0 =
+// the launchers' own rule).
+bool tuned_auto(const vx_batch *b) { return b->mode == VX_BATCH_AUTO && b->tuned; }
+int32_t fill_split_of(const vx_batch *b) {
+    return tuned_auto(b) ? b->tuned_fill_split : b->fill_split;
+}
+int32_t emit_split_of(const vx_batch *b) {
+    return tuned_auto(b) ? b->tuned_emit_split : b->emit_split;
+}
+
 hipError_t enqueue_fill(vx_batch *b) {
     vx_ctx *ctx = b->ctx;
     return launch_fill(b->fill_dev, b->order_dev, b->n, b->max_edge, ctx->terrains_dev,
-                       ctx->lut_dev, b->overflow + 1, ctx->stream);
+                       ctx->lut_dev, b->overflow + 1, fill_split_of(b), ctx->stream);
 }
 hipError_t enqueue_count(vx_batch *b) {
     return launch_count(b->mesh_dev, b->n, b->max_edge, b->counts, b->slabs, b->ctx->stream);
@@ -160,7 +171,7 @@ hipError_t enqueue_scan(vx_batch *b) {
 hipError_t enqueue_emit(vx_batch *b) {
     return launch_emit(b->mesh_dev, b->emit_order, b->n, b->n_items, b->max_edge, b->bases,
                        b->counts, b->slabs, b->out, b->ctx->tables_dev, b->overflow,
-                       b->ctx->stream);
+                       emit_split_of(b), b->ctx->stream);
 }
 
 // The form a run takes.  VX_BATCH_AUTO runs the staged form unless the
@@ -268,8 +279,12 @@ int fetch_results(vx_batch *b) {
 // Shape of a batch for the tuning cache: chunk count bucket (power of two),
 // largest edge, uniform or ragged, fused form eligible, has neighbours.
 using TuneKey = std::tuple<int, int, int, int, int>;
+bool fill_split_ok(int32_t v) { return v == 0 || v == 1 || v == 2 || v == 4 || v == 8; }
+bool emit_split_ok(int32_t v) { return v >= 0 && v <= 4; }
+
 struct TuneChoice {
     int32_t mode, grid, ratio;  // ratio: pipelined roles (0 = default)
+    int32_t fill_split = 0, emit_split = 0;  // staged workgroups per chunk / item (0 = rule)
 };
 std::map<TuneKey, TuneChoice> g_tune;
 std::mutex g_tune_mu;
@@ -286,7 +301,7 @@ TuneKey tune_key(const vx_batch *b) {
 
 // The cache persists across processes when VX_TUNE_CACHE names a file:
 // one line per shape, "bucket max_edge uniform fused_ok has_nb mode grid
-// ratio".
+// ratio fill_split emit_split".
 void tune_load() {
     if (g_tune_loaded) return;
     g_tune_loaded = true;
@@ -294,16 +309,19 @@ void tune_load() {
     if (!path) return;
     FILE *f = fopen(path, "r");
     if (!f) return;
-    // line by line: a line of the earlier seven-field form (no ratio) reads
-    // as the default split; a line that is neither form is skipped
+    // line by line: lines of the earlier seven- and eight-field forms (no
+    // role ratio, no splits) read with the defaults; a line that is none of
+    // the forms, or holds a value no setter accepts, is skipped
     char line[256];
     while (fgets(line, sizeof line, f)) {
-        int a, c, d, e, h, m, g, r = 0;
-        const int k = sscanf(line, "%d %d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g, &r);
-        if (k != 7 && k != 8) continue;
+        int a, c, d, e, h, m, g, r = 0, fs = 0, es = 0;
+        const int k = sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g, &r,
+                             &fs, &es);
+        if (k != 7 && k != 8 && k != 10) continue;
         const bool mode_ok = m == VX_BATCH_STAGED || m == VX_BATCH_FUSED || m == VX_BATCH_PIPELINED;
         const bool ratio_ok = r == 0 || (r >= 2 && r <= kPipeRatioMax);
-        if (mode_ok && ratio_ok && g >= 0) g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g, r};
+        if (mode_ok && ratio_ok && g >= 0 && fill_split_ok(fs) && emit_split_ok(es))
+            g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g, r, fs, es};
     }
     fclose(f);
 }
@@ -314,9 +332,10 @@ void tune_save() {
     FILE *f = fopen(path, "w");
     if (!f) return;
     for (auto &kv : g_tune)
-        fprintf(f, "%d %d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
-                std::get<2>(kv.first), std::get<3>(kv.first), std::get<4>(kv.first),
-                kv.second.mode, kv.second.grid, kv.second.ratio);
+        fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", std::get<0>(kv.first),
+                std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first),
+                std::get<4>(kv.first), kv.second.mode, kv.second.grid, kv.second.ratio,
+                kv.second.fill_split, kv.second.emit_split);
     fclose(f);
 }
 
@@ -561,6 +580,8 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
             b->tuned_mode = it->second.mode;
             b->tuned_grid = it->second.grid;
             b->tuned_ratio = it->second.ratio;
+            b->tuned_fill_split = it->second.fill_split;
+            b->tuned_emit_split = it->second.emit_split;
         }
     }
     *out = b;
@@ -610,6 +631,18 @@ extern "C" int vx_batch_set_roles(vx_batch *b, int32_t ratio) {
     return VX_OK;
 }
 
+extern "C" int vx_batch_set_splits(vx_batch *b, int32_t fill_parts, int32_t emit_parts) {
+    if (!b || !fill_split_ok(fill_parts) || !emit_split_ok(emit_parts)) return VX_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    if (fill_parts != b->fill_split || emit_parts != b->emit_split) {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        drop_graph(b);
+        b->fill_split = fill_parts;
+        b->emit_split = emit_parts;
+    }
+    return VX_OK;
+}
+
 extern "C" int vx_batch_get_mode(vx_batch *b, int32_t *mode) {
     if (!b || !mode) return VX_ERR_INVALID;
     *mode = form_of(b);
@@ -635,7 +668,7 @@ extern "C" int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *b) {
     const int n = b->n;
     for (int i = 0; i < n; i++)
         VX_HIP(ctx, launch_fill(b->fill_dev + i, nullptr, 1, b->reqs[i].edge, ctx->terrains_dev,
-                                ctx->lut_dev, b->overflow + 1, ctx->stream));
+                                ctx->lut_dev, b->overflow + 1, 0, ctx->stream));
     auto slabs_of = [&](int i) { return b->slabs ? b->slabs + (size_t)kSlabs * i : nullptr; };
     for (int i = 0; i < n; i++)
         VX_HIP(ctx, launch_count(b->mesh_dev + i, 1, b->reqs[i].edge, b->counts + i, slabs_of(i),
@@ -644,7 +677,7 @@ extern "C" int vx_batch_run_per_chunk(vx_ctx *ctx, vx_batch *b) {
     for (int i = 0; i < n; i++)
         VX_HIP(ctx, launch_emit(b->mesh_dev + i, nullptr, 1, 0, b->reqs[i].edge, b->bases + i,
                                 b->counts + i, slabs_of(i), b->out, ctx->tables_dev, b->overflow,
-                                ctx->stream));
+                                0, ctx->stream));
     b->ran = true;
     b->results = false;
     return VX_OK;
@@ -865,6 +898,23 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     std::vector<TuneChoice> cands;
     cands.push_back({VX_BATCH_STAGED, 0, 0});
+    // staged tile shapes for batches small enough that the launchers split
+    // chunks (the fill, under kFillGrid chunks) or items (the emit, under
+    // kEmitGrid items) over several workgroups: half and twice the fill's
+    // rule, and one or four (two when the rule gives four) emit workgroups
+    // per item, each beside the other stage's rule
+    {
+        const int32_t items = emit_items_of(b->n, b->n_items, b->max_edge);
+        const int hf = fill_parts_for(b->n, b->max_edge), he = emit_parts_for(items);
+        if (b->n < kFillGrid) {
+            cands.push_back({VX_BATCH_STAGED, 0, 0, hf / 2, 0});
+            if (hf * 2 <= (b->max_edge > 32 ? 8 : 4)) cands.push_back({VX_BATCH_STAGED, 0, 0, hf * 2, 0});
+        }
+        if (items < kEmitGrid) {
+            if (he != 1) cands.push_back({VX_BATCH_STAGED, 0, 0, 0, 1});
+            cands.push_back({VX_BATCH_STAGED, 0, 0, 0, he == 4 ? 2 : 4});
+        }
+    }
     if (b->fused_ok) {
         cands.push_back({VX_BATCH_FUSED, 0, 0});
         for (int per = 3; per <= 4; per++)
@@ -894,6 +944,8 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
             b->mode = cands[c].mode;
             b->grid = cands[c].grid;
             b->pipe_ratio = cands[c].ratio;
+            b->fill_split = cands[c].fill_split;
+            b->emit_split = cands[c].emit_split;
             if ((st = capture_graph(b))) break;
             hipError_t e = hipGraphLaunch(b->graph_exec, ctx->stream);  // warm-up
             if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
@@ -916,6 +968,8 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
             r.cand_mode[c] = cands[c].mode;
             r.cand_grid[c] = cands[c].grid;
             r.cand_ratio[c] = cands[c].ratio;
+            r.cand_fill_split[c] = cands[c].fill_split;
+            r.cand_emit_split[c] = cands[c].emit_split;
             r.cand_ms[c] = tmin[c];
             r.n_candidates = (int32_t)c + 1;
         }
@@ -927,16 +981,21 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
     b->mode = VX_BATCH_AUTO;
     b->grid = 0;
     b->pipe_ratio = 0;
+    b->fill_split = b->emit_split = 0;
     b->tuned = true;
     b->tuned_mode = bestc.mode;
     b->tuned_grid = bestc.grid;
     b->tuned_ratio = bestc.ratio;
+    b->tuned_fill_split = bestc.fill_split;
+    b->tuned_emit_split = bestc.emit_split;
     b->ran = true;
     b->results = false;
     if (st) return st;
     r.mode = bestc.mode;
     r.grid = bestc.grid;
     r.ratio = bestc.ratio;
+    r.fill_split = bestc.fill_split;
+    r.emit_split = bestc.emit_split;
     r.best_ms = best;
     {
         std::lock_guard<std::mutex> tg(g_tune_mu);

@@ -155,8 +155,11 @@ struct vx_batch {
     int32_t mode = 0;               // VX_BATCH_* requested
     int32_t grid = 0;               // fused workgroups (0 = one per chunk)
     bool tuned = false;             // an autotuned choice applies in VX_BATCH_AUTO
-    int32_t tuned_mode = 0, tuned_grid = 0, tuned_ratio = 0;
+    int32_t tuned_mode = 0, tuned_grid = 0, tuned_ratio = 0, tuned_fill_split = 0,
+            tuned_emit_split = 0;
     int32_t pipe_ratio = 0;         // pipelined roles: one mesher in pipe_ratio (0 = default)
+    int32_t fill_split = 0;         // staged fill workgroups per chunk (0 = the launcher's rule)
+    int32_t emit_split = 0;         // staged emit workgroups per item (0 = the launcher's rule)
     bool fused_ok = false;          // fused form eligible
     bool pipe_ok = false;           // pipelined form eligible (fused_ok, edges <= 32)
     bool fill_counts = false;       // the staged fill also counts (no count kernel)

@@ -167,11 +167,32 @@ inline int words_per_row(int edge) { return (edge + 31) / 32; }
 // null for 0..n-1.  An entry outside [0, n) makes its workgroup do nothing
 // and sets a sticky error word: `error` for the fill, overflow[1] for the
 // emit, whose overflow[0] is the arena-overflow flag the scan clears.
+// Workgroups per chunk of the fill: one for batches of kFillGrid chunks or
+// more; below that 2, 4 or 8 so that the grid still covers about kFillGrid
+// workgroups (8 only for batches of at most kFillGrid / 8 chunks that hold
+// 64^3 chunks).  `hint` (1, 2, 4 or 8; 0 = this rule) replaces the rule
+// for the batches it would split.
+inline int fill_parts_for(int32_t n, int32_t max_edge, int32_t hint = 0) {
+    if (n >= kFillGrid) return 1;
+    if (hint > 0) return hint;
+    return 2 * n >= kFillGrid ? 2 : (max_edge > 32 && 8 * n <= kFillGrid) ? 8 : 4;
+}
+// Workgroups per emit item: one for kEmitGrid items or more; below that up to
+// 4, so that the grid still covers about kEmitGrid workgroups.  `hint` (1..4;
+// 0 = this rule) replaces the rule for the batches it would split.
+inline int emit_parts_for(int32_t items, int32_t hint = 0) {
+    if (items >= kEmitGrid) return 1;
+    if (hint > 0) return hint;
+    const int p = (kEmitGrid + items - 1) / items;
+    return p < 4 ? p : 4;
+}
+// parts_hint: fill_parts_for's hint.
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const TerrainDev *terrains, const GradLut *lut, uint32_t *error,
-                       hipStream_t s);
+                       int32_t parts_hint, hipStream_t s);
 // True iff launch_fill gives each chunk one workgroup (fill_kernel), so that
-// FillWork.count_out is honoured for every chunk of the batch.
+// FillWork.count_out is honoured for every chunk of the batch (whatever the
+// hint).
 inline bool fill_whole_chunks(int32_t n) { return n >= kFillGrid; }
 // Batches that hold 64^3 chunks emit by items: each 64^3 chunk is kSlabs
 // y-slabs of kSlabLayers layers (face counts per slab in slabs[kSlabs * i
@@ -189,10 +210,16 @@ hipError_t launch_pack_one(const FillWork &w, hipStream_t s);
 hipError_t launch_count_one(const MeshWork &w, uint32_t *count, hipStream_t s);
 hipError_t launch_emit_one(const MeshWork &w, uint64_t base, uint32_t faces, MeshOut out,
                            const MeshTables *tables, uint32_t *overflow, hipStream_t s);
+// parts_hint: emit_parts_for's hint.
 hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t n_items,
                        int32_t max_edge, const uint64_t *bases, const uint32_t *counts,
                        const uint32_t *slabs, MeshOut out, const MeshTables *tables,
-                       uint32_t *overflow, hipStream_t s);
+                       uint32_t *overflow, int32_t parts_hint, hipStream_t s);
+// Emit items of a batch: its chunks, or for batches that hold 64^3 chunks
+// kSlabs per 64^3 chunk and one per other chunk.
+inline int32_t emit_items_of(int32_t n, int32_t n_items, int32_t max_edge) {
+    return max_edge <= 32 ? n : n_items;
+}
 hipError_t launch_fused(const FusedWork *work, int32_t n, int32_t max_edge,
                         const TerrainDev *terrains, const GradLut *lut, const MeshTables *tables,
                         FusedCtl ctl, MeshOut out, const CullJob &cull, int32_t grid,

@@ -2124,16 +2124,14 @@ __global__ void __launch_bounds__(kBlock) draw_write_kernel(Planes planes,
 // --------------------------------------------------------------- launchers
 hipError_t launch_fill(const FillWork *work, const int32_t *order, int32_t n, int32_t max_edge,
                        const TerrainDev *terrains, const GradLut *lut, uint32_t *error,
-                       hipStream_t s) {
+                       int32_t parts_hint, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    // small batches: 2, 4 or 8 workgroups per chunk, so the grid still
-    // covers about kFillGrid workgroups (density terrains with up to 4
-    // octaves split their rows; other chunks are filled by the first of
-    // their workgroups); 8 only for batches of at most 128 chunks that hold
-    // 64^3 chunks, whose parts then fill 8 rows each
-    const int parts = n >= kFillGrid ? 1
-                      : 2 * n >= kFillGrid ? 2
-                      : (max_edge > 32 && 8 * n <= kFillGrid) ? 8 : 4;
+    if (parts_hint != 0 && parts_hint != 1 && parts_hint != 2 && parts_hint != 4 && parts_hint != 8)
+        return hipErrorInvalidValue;
+    // small batches: several workgroups per chunk (density terrains with up
+    // to 4 octaves split their rows; other chunks are filled by the first of
+    // their workgroups; a chunk under 64^3 given 8 uses the first 4)
+    const int parts = fill_parts_for(n, max_edge, parts_hint);
     if (parts == 1) {
         if (max_edge <= 16) fill_kernel<16><<<n, kBlock, 0, s>>>(work, order, n, terrains, lut, error);
         else if (max_edge <= 32) fill_kernel<32><<<n, kBlock, 0, s>>>(work, order, n, terrains, lut, error);
@@ -2199,16 +2197,17 @@ hipError_t launch_emit_one(const MeshWork &w, uint64_t base, uint32_t faces, Mes
 hipError_t launch_emit(const MeshWork *work, const int32_t *order, int32_t n, int32_t n_items,
                        int32_t max_edge, const uint64_t *bases, const uint32_t *counts,
                        const uint32_t *slabs, MeshOut out, const MeshTables *tables,
-                       uint32_t *overflow, hipStream_t s) {
+                       uint32_t *overflow, int32_t parts_hint, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (parts_hint < 0 || parts_hint > 4) return hipErrorInvalidValue;
     // items: chunks, or for batches that hold 64^3 chunks the scan's emit
     // items (kSlabs codes per chunk without an order)
     const int items = max_edge <= 32 ? n : order ? n_items : kSlabs * n;
     if (items <= 0) return hipErrorInvalidValue;
-    // small batches: up to 4 workgroups per item, so the grid still covers
+    // small batches: several workgroups per item, so the grid still covers
     // about a full GPU of workgroups and an item's tiles are emitted side by
     // side
-    const int parts = items >= kEmitGrid ? 1 : std::min(4, (kEmitGrid + items - 1) / items);
+    const int parts = emit_parts_for(items, parts_hint);
     const int grid = items * parts;
     if (max_edge <= 16)
         emit_kernel<16, kBlock><<<grid, kBlock, 0, s>>>(work, order, bases, counts, out, tables,

@@ -415,7 +415,7 @@ int admit(vx_server *S) {
     }
     if (e == hipSuccess)
         e = launch_fill(F->fill_dev, F->order_dev, n, F->max_edge, ctx->terrains_dev, ctx->lut_dev,
-                        F->overflow + 1, s);
+                        F->overflow + 1, 0, s);
     if (e == hipSuccess) e = launch_count(F->mesh_dev, n, F->max_edge, F->counts, F->slabs, s);
     // the scan rewrites the order array (the fill has consumed it) as the
     // emission order, largest chunks first (emit items when slabbed)
@@ -499,7 +499,7 @@ int advance(vx_server *S, Flight *F) {
         }
         hipError_t e = launch_emit(F->mesh_dev, F->order_dev, F->n, F->n_items, F->max_edge,
                                    F->bases, F->counts, F->slabs, F->out, ctx->tables_dev,
-                                   F->overflow, sl.stream);
+                                   F->overflow, 0, sl.stream);
         unsigned char *back = sl.pinned + sl.pinned_bytes - 256 - up256((size_t)S->max_batch * 12);
         if (e == hipSuccess)
             e = hipMemcpyAsync(back, F->counts, (size_t)F->n * 4, hipMemcpyDeviceToHost, sl.stream);

@@ -109,7 +109,9 @@ class TuneResult(C.Structure):
     _fields_ = [("mode", C.c_int32), ("grid", C.c_int32), ("best_ms", C.c_float),
                 ("n_candidates", C.c_int32), ("cand_mode", C.c_int32 * 8),
                 ("cand_grid", C.c_int32 * 8), ("cand_ms", C.c_float * 8),
-                ("ratio", C.c_int32), ("cand_ratio", C.c_int32 * 8)]
+                ("ratio", C.c_int32), ("cand_ratio", C.c_int32 * 8),
+                ("fill_split", C.c_int32), ("emit_split", C.c_int32),
+                ("cand_fill_split", C.c_int32 * 8), ("cand_emit_split", C.c_int32 * 8)]
 
 
 class WorldDesc(C.Structure):
@@ -184,6 +186,7 @@ _SIGS = {
     "vx_batch_set_mode": (C.c_int, [P, C.c_int32]),
     "vx_batch_set_grid": (C.c_int, [P, C.c_int32]),
     "vx_batch_set_roles": (C.c_int, [P, C.c_int32]),
+    "vx_batch_set_splits": (C.c_int, [P, C.c_int32, C.c_int32]),
     "vx_batch_get_mode": (C.c_int, [P, C.POINTER(C.c_int32)]),
     "vx_batch_run_per_chunk": (C.c_int, [P, P]),
     "vx_batch_wait": (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
@@ -593,6 +596,12 @@ class Batch:
         """Pipelined form: one mesher workgroup in `ratio` (2..8; 0 = default)."""
         self.ctx.check(self.ctx.L.vx_batch_set_roles(self.h, ratio), "vx_batch_set_roles")
 
+    def set_splits(self, fill_parts: int, emit_parts: int):
+        """Staged form, small batches: fill workgroups per chunk (1, 2, 4, 8)
+        and emit workgroups per item (1..4); 0 = the launchers' rule."""
+        self.ctx.check(self.ctx.L.vx_batch_set_splits(self.h, fill_parts, emit_parts),
+                       "vx_batch_set_splits")
+
     @property
     def mode(self) -> int:
         m = C.c_int32()
@@ -643,9 +652,10 @@ class Batch:
                        "vx_batch_autotune")
         names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged", BATCH_PIPELINED: "pipelined"}
         return {"mode": names.get(r.mode, r.mode), "grid": r.grid, "ratio": r.ratio,
-                "best_ms": round(r.best_ms, 4),
+                "splits": [r.fill_split, r.emit_split], "best_ms": round(r.best_ms, 4),
                 "candidates": [{"mode": names.get(r.cand_mode[i], r.cand_mode[i]),
                                 "grid": r.cand_grid[i], "ratio": r.cand_ratio[i],
+                                "splits": [r.cand_fill_split[i], r.cand_emit_split[i]],
                                 "ms": round(r.cand_ms[i], 4)}
                                for i in range(r.n_candidates)]}
 
