@@ -1,7 +1,7 @@
 // Host-side memory check of libvoxel_hip: drives every subsystem through the
 // C ABI -- contexts, chunks and meshes (retained and released in every
-// order), batches in both forms with arena growth, chunks that outlive their
-// batch, the server under an HBM budget (eviction, destruction with work
+// order), batches in every form and launch shape with arena growth, the
+// autotuner and its cache file, chunks that outlive their batch, the server under an HBM budget (eviction, destruction with work
 // in flight, handles that outlive it), the world tick loop, frustum
 // masks and draw lists -- so that AddressSanitizer and UBSan, compiled into
 // the host code only, see its allocations, reference counts and copies.
@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -119,6 +120,29 @@ static int batches(vx_ctx *ctx, int32_t tid) {
     CHECK(vx_batch_run(ctx, b));
     CHECK(vx_batch_wait(ctx, b, &t_fused));
     EXPECT(t_staged == t_fused && t_staged > 1000);
+    // launch shapes: pipelined roles and grid, staged tile shapes, and the
+    // autotuner over all of them (it writes the VX_TUNE_CACHE file)
+    uint64_t t_shape = 0;
+    CHECK(vx_batch_set_mode(b, VX_BATCH_PIPELINED));
+    CHECK(vx_batch_set_grid(b, 7));
+    CHECK(vx_batch_set_roles(b, 4));
+    CHECK(vx_batch_run(ctx, b));
+    CHECK(vx_batch_wait(ctx, b, &t_shape));
+    EXPECT(t_shape == t_staged);
+    CHECK(vx_batch_set_mode(b, VX_BATCH_STAGED));
+    CHECK(vx_batch_set_splits(b, 8, 1));
+    CHECK(vx_batch_run(ctx, b));
+    CHECK(vx_batch_wait(ctx, b, &t_shape));
+    EXPECT(t_shape == t_staged);
+    EXPECT(vx_batch_set_splits(b, 3, 0) == VX_ERR_INVALID);
+    EXPECT(vx_batch_set_roles(b, 1) == VX_ERR_INVALID);
+    CHECK(vx_batch_set_mode(b, VX_BATCH_AUTO));
+    vx_tune_result tr{};
+    CHECK(vx_batch_autotune(ctx, b, 1, &tr));
+    EXPECT(tr.n_candidates >= 5 && tr.best_ms > 0.0f);
+    CHECK(vx_batch_run(ctx, b));
+    CHECK(vx_batch_wait(ctx, b, &t_shape));
+    EXPECT(t_shape == t_staged);
     uint64_t first = 0, count = 0;
     CHECK(vx_batch_chunk_faces(ctx, b, n - 1, &first, &count));
     EXPECT(first + count == t_staged);
@@ -269,6 +293,21 @@ int main() {
         std::fprintf(stderr, "no HIP device\n");
         return 2;
     }
+    // the autotuner's cache file, read when the first batch is created: an
+    // entry of each form and lines of every accepted and refused shape
+    char cache[] = "/tmp/vx_host_check_tune_XXXXXX";
+    const int fd = mkstemp(cache);
+    if (fd < 0) return 1;
+    const char lines[] =
+        "not an entry\n"
+        "6 32 1 1 1 1 0\n"
+        "7 32 1 1 1 3 0 4\n"
+        "8 32 1 1 1 2 0 0 2 3\n"
+        "9 32 1 1 1 2 0 0 3 0\n"
+        "10 32 1 1 1 3 0 1\n";
+    if (write(fd, lines, sizeof lines - 1) != (ssize_t)(sizeof lines - 1)) return 1;
+    close(fd);
+    setenv("VX_TUNE_CACHE", cache, 1);
     vx_ctx *ctx = nullptr;
     vx_ctx_desc d{0, 0, 0};
     CHECK(vx_ctx_create(&d, &ctx));
@@ -295,6 +334,7 @@ int main() {
     // running the HIP runtime's own static teardown: the sanitizer runtime's
     // device-memory bookkeeping cannot release blocks once that teardown has
     // unloaded the HSA runtime, which is no property of this library.
+    unlink(cache);
     const int leaks = __lsan_do_recoverable_leak_check();
     if (leaks) {
         std::fprintf(stderr, "leak check failed\n");
