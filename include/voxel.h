@@ -362,9 +362,11 @@ int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
  * and one mesher in two, three or four) over `reps` graph replays each, in
  * two rounds (each candidate's faster round counts), keeps the fastest for
  * the batch, and caches the choice for batches of the same shape
- * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  With
- * VX_TUNE_CACHE set to a file path the cache persists across processes.
- * Batches created later in VX_BATCH_AUTO use a cached choice. */
+ * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  The
+ * batch is left in VX_BATCH_AUTO running the choice; shapes set before with
+ * vx_batch_set_grid / _roles / _splits are cleared.  With VX_TUNE_CACHE set
+ * to a file path the cache persists across processes.  Batches created later
+ * in VX_BATCH_AUTO use a cached choice. */
 #define VX_TUNE_MAX_CANDIDATES 8
 typedef struct vx_tune_result {
     int32_t mode;  /* chosen VX_BATCH_FUSED / VX_BATCH_STAGED / VX_BATCH_PIPELINED */
