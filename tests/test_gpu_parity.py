@@ -520,6 +520,37 @@ def test_batch_ragged(ctx, oracle_mod, mode):
     batch_vs_oracle(ctx, oracle_mod, coords, edges, terrains, np.full((n, 4), -1), mode=mode)
 
 
+def test_batch_ragged_autotune_keeps_results(ctx, oracle_mod):
+    """The autotuner on a small ragged batch holding 64^3 chunks (slab emit
+    items, an 8-way fill split by rule): staged tile shapes and the fused
+    form are timed, and the tuned batch's ids and meshes are the oracle's,
+    byte for byte."""
+    say("batch ragged autotune")
+    rng = np.random.default_rng(11)
+    n = 48
+    edges = rng.choice([16, 32, 64], size=n)
+    cls = rng.choice(3, size=n, p=[0.3, 0.1, 0.6])
+    terrains = []
+    for e, k in zip(edges, cls):
+        if k == 0:
+            terrains.append(V.Terrain(kind=V.TERRAIN_AIR))
+        elif k == 1:
+            terrains.append(V.Terrain(kind=V.TERRAIN_SOLID))
+        else:
+            terrains.append(C2)
+    coords = [(i % 8, 0, i // 8) for i in range(n)]
+    nb = np.full((n, 4), -1)
+    fmt = V.FMT_VERTICES | V.FMT_INSTANCES
+    b = V.Batch(ctx, coords, edges, [ctx.terrain(t) for t in terrains], nb, formats=fmt)
+    r = b.autotune(reps=2)
+    shapes = {(c["mode"], tuple(c["splits"])) for c in r["candidates"]}
+    assert ("staged", (0, 0)) in shapes and ("fused", (0, 0)) in shapes, r
+    assert any(m == "staged" and f for m, (f, _) in shapes), r
+    assert any(m == "staged" and e for m, (_, e) in shapes), r
+    check_batch(b, oracle_mod, coords, edges, terrains, nb, fmt)
+    b.close()
+
+
 def test_batch_fill_counts_against_resident_neighbours(ctx, oracle_mod):
     """1,024 chunks of 16^3 (one fill workgroup each) with no neighbour
     inside the batch: the staged fill counts the faces itself, so the batch
