@@ -341,24 +341,26 @@ def test_batch_autotune_keeps_results(ctx, oracle_mod):
 
 @pytest.mark.parametrize("splits", [(1, 1), (2, 4), (4, 2), (8, 3), (0, 1), (1, 0)])
 @pytest.mark.parametrize("edge", [32, 64])
-def test_batch_staged_tile_shapes(ctx, oracle_mod, splits, edge):
+@pytest.mark.parametrize("terrain", [C2, C1], ids=["density", "heightmap"])
+def test_batch_staged_tile_shapes(ctx, oracle_mod, splits, edge, terrain):
     """The staged form at set tile shapes (vx_batch_set_splits): fill
-    workgroups per chunk splitting the density rows, emit workgroups per
-    item splitting its tiles, on a 3x3 grid with in-grid neighbours; ids and
+    workgroups per chunk splitting the density rows (a heightmap chunk is
+    filled whole by the first of its workgroups), emit workgroups per item
+    splitting its tiles, on a 3x3 grid with in-grid neighbours; ids and
     meshes byte for byte (at edge 32 a fill split of 8 uses 4 of them).
     Values outside 1, 2, 4, 8 and 1..4 are refused."""
-    say("batch tile shapes", splits, edge)
+    say("batch tile shapes", splits, edge, terrain.kind)
     coords = [(x - 1, 0, z + 2) for x in range(3) for z in range(3)]
     nb = V.grid_neighbours(3, 3)
     fmt = V.FMT_VERTICES | V.FMT_INSTANCES
-    b = V.Batch(ctx, coords, edge, ctx.terrain(C2), nb, formats=fmt)
+    b = V.Batch(ctx, coords, edge, ctx.terrain(terrain), nb, formats=fmt)
     b.set_mode(V.BATCH_STAGED)
     for bad in ((3, 0), (16, 0), (-1, 0), (0, 5), (0, -1)):
         with pytest.raises(V.VoxelError) as e:
             b.set_splits(*bad)
         assert e.value.status == V.ERR_INVALID
     b.set_splits(*splits)
-    check_batch(b, oracle_mod, coords, [edge] * 9, [C2] * 9, nb, fmt)
+    check_batch(b, oracle_mod, coords, [edge] * 9, [terrain] * 9, nb, fmt)
     b.close()
 
 
