@@ -30,19 +30,25 @@ def check_sample(O, b, t, coords, edges, nb, idx, verts, idxs):
         assert (idxs[6 * first:6 * (first + cnt)] == ix).all(), i
 
 
-@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED])
+@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED, "autotuned"])
 def test_config2_full_grid(ctx, oracle_mod, mode):
     """4096 chunks of 32^3 (64 x 64 grid, seed 7): every chunk's face count
     equals the oracle's, bases are the exclusive prefix in request order, 16
     seeded chunks match byte for byte with oracle meshes built one by one,
     and the whole vertex and index buffers (11.2 M faces, 1.17 GB) equal the
-    oracle driver's, laid out at the same bases."""
+    oracle driver's, laid out at the same bases.  "autotuned": the path the
+    benchmark times, vx_batch_autotune and then VX_BATCH_AUTO."""
     O = oracle_mod
     say("config 2 full", mode)
     coords = np.array([(x, 0, z) for x in range(64) for z in range(64)], np.int32)
     nb = V.grid_neighbours(64, 64)
     b = V.Batch(ctx, coords, 32, ctx.terrain(C2), nb, formats=V.FMT_VERTICES)
-    b.set_mode(mode)
+    if mode == "autotuned":
+        b.run()
+        b.wait()
+        b.autotune(2)
+    else:
+        b.set_mode(mode)
     if mode == V.BATCH_PIPELINED:
         b.set_grid(4 * 256)  # persistent: four workgroups per CU
     b.run()
