@@ -353,8 +353,9 @@ int vx_batch_draw_list(vx_ctx *ctx, vx_batch *batch, const vx_frustum *f, uint32
  * ms[1..3] = 0;
  * ms[4] = whole run replayed from the graph. */
 int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
-/* Autotuner: times every candidate execution form and launch shape of this
- * batch (staged; for small batches also staged with other tile shapes,
+/* Autotuner: runs the batch once (sizing its mesh arena, so every candidate
+ * is timed on the whole output), then times every candidate execution form
+ * and launch shape of this batch (staged; for small batches also staged with other tile shapes,
  * vx_batch_set_splits: half and twice the fill's workgroups per chunk, one
  * and four (or two) emit workgroups per item; fused with one workgroup per
  * chunk and with 3 or 4 persistent workgroups per CU; pipelined with one

@@ -69,13 +69,15 @@ def test_config2_full_grid(ctx, oracle_mod, mode):
     assert np.array_equal(idxs, oidx)
 
 
-@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED])
+@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED, "autotuned"])
 def test_config3_full_ragged(ctx, oracle_mod, mode):
     """2048 chunks, edges uniform over {16, 32, 64}, classes 30% air / 10%
     solid / 60% 4-octave heightmap surface (one seeded stream, seed 11), no
     neighbours: every chunk's faces byte for byte, prefix order.  The
     15.0 M faces exceed the default arena (4 N^2 per chunk, 14.5 M), so the
-    run also exercises the grow-and-re-emit path after either form."""
+    run also exercises the grow-and-re-emit path after either form.
+    "autotuned": vx_batch_autotune on the fresh batch (its candidates
+    outgrow the arena too), then VX_BATCH_AUTO."""
     O = oracle_mod
     say("config 3 full", mode)
     rng = np.random.default_rng(11)
@@ -99,7 +101,10 @@ def test_config3_full_ragged(ctx, oracle_mod, mode):
     coords = np.array([(i % 64, 0, i // 64) for i in range(n)], np.int32)
     nb = np.full((n, 4), -1, np.int32)
     b = V.Batch(ctx, coords, edges, tids, nb, formats=V.FMT_VERTICES)
-    b.set_mode(mode)
+    if mode == "autotuned":
+        b.autotune(2)
+    else:
+        b.set_mode(mode)
     b.run()
     total = b.wait()
     assert total == 15027984

@@ -894,6 +894,15 @@ extern "C" int vx_batch_device_buffers(vx_batch *b, void **inst, void **vert, vo
 extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune_result *res) {
     if (!ctx || !b || b->ctx != ctx || reps <= 0) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    {
+        // one run in the current form first: it grows the mesh arena if the
+        // batch's faces outgrow it, so that no candidate is timed on a run
+        // that skips the faces beyond the arena
+        uint64_t faces = 0;
+        int st0 = vx_batch_run(ctx, b);
+        if (!st0) st0 = vx_batch_wait(ctx, b, &faces);
+        if (st0) return st0;
+    }
     int cus = 1;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     std::vector<TuneChoice> cands;
