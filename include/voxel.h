@@ -301,11 +301,19 @@ int vx_batch_destroy(vx_batch *batch);
  *                    run side by side on every CU; neighbour planes come
  *                    from the neighbours' fills.  Eligible as VX_BATCH_FUSED,
  *                    for edges up to 32.
+ *   VX_BATCH_SPLIT   the staged kernels over two halves of the batch on two
+ *                    streams of one graph: the first half (and the later
+ *                    chunks it borders) is filled, then its count, scan and
+ *                    emit run beside the second half's fill; the second
+ *                    half's scan continues from the first half's total.
+ *                    Edges up to 32; a batch with no in-batch neighbour
+ *                    needs 2,048 chunks or more (whole-chunk fills).
  * Every form produces identical bytes. */
 #define VX_BATCH_AUTO 0
 #define VX_BATCH_FUSED 1
 #define VX_BATCH_STAGED 2
 #define VX_BATCH_PIPELINED 3
+#define VX_BATCH_SPLIT 4
 int vx_batch_set_mode(vx_batch *batch, int32_t mode);
 /* Workgroups of the persistent launch in VX_BATCH_FUSED / VX_BATCH_PIPELINED
  * (0 = one per chunk; fewer = persistent workgroups that keep taking chunk
@@ -360,7 +368,8 @@ int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
  * and four (or two) emit workgroups per item; fused with one workgroup per
  * chunk and with 3 or 4 persistent workgroups per CU; pipelined with one
  * workgroup per chunk and one mesher in three, and with 8 workgroups per CU
- * and one mesher in two, three or four) over `reps` graph replays each, in
+ * and one mesher in two, three or four; split in two halves on two streams)
+ * over `reps` graph replays each, in
  * two rounds (each candidate's faster round counts), keeps the fastest for
  * the batch, and caches the choice for batches of the same shape
  * (chunk-count bucket, largest edge, uniform or ragged, neighbours).  The
@@ -368,9 +377,9 @@ int vx_batch_profile(vx_ctx *ctx, vx_batch *batch, int32_t reps, float ms[5]);
  * vx_batch_set_grid / _roles / _splits are cleared.  With VX_TUNE_CACHE set
  * to a file path the cache persists across processes.  Batches created later
  * in VX_BATCH_AUTO use a cached choice. */
-#define VX_TUNE_MAX_CANDIDATES 8
+#define VX_TUNE_MAX_CANDIDATES 12
 typedef struct vx_tune_result {
-    int32_t mode;  /* chosen VX_BATCH_FUSED / VX_BATCH_STAGED / VX_BATCH_PIPELINED */
+    int32_t mode;  /* chosen VX_BATCH_FUSED / _STAGED / _PIPELINED / _SPLIT */
     int32_t grid;  /* chosen launch shape (vx_batch_set_grid) */
     float best_ms;
     int32_t n_candidates;

@@ -30,7 +30,8 @@ def check_sample(O, b, t, coords, edges, nb, idx, verts, idxs):
         assert (idxs[6 * first:6 * (first + cnt)] == ix).all(), i
 
 
-@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED, "autotuned"])
+@pytest.mark.parametrize("mode", [V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED, V.BATCH_SPLIT,
+                                  "autotuned"])
 def test_config2_full_grid(ctx, oracle_mod, mode):
     """4096 chunks of 32^3 (64 x 64 grid, seed 7): every chunk's face count
     equals the oracle's, bases are the exclusive prefix in request order, 16

@@ -225,11 +225,11 @@ def apply_mode(b, mode):
     """mode: a VX_BATCH_* value or (mode, grid[, ratio]), grid = persistent
     workgroups, ratio = one pipelined mesher workgroup in `ratio`."""
     m, g, r = (tuple(mode) + (0,))[:3] if isinstance(mode, tuple) else (mode, 0, 0)
-    if m == V.BATCH_PIPELINED and int(np.max(b.edges)) > 32:
+    if m in (V.BATCH_PIPELINED, V.BATCH_SPLIT) and int(np.max(b.edges)) > 32:
         with pytest.raises(V.VoxelError) as e:
             b.set_mode(m)
         assert e.value.status == V.ERR_INVALID
-        pytest.skip("the pipelined form takes edges up to 32 (refused as specified)")
+        pytest.skip("the pipelined and split forms take edges up to 32 (refused as specified)")
     b.set_mode(m)
     b.set_grid(g)
     b.set_roles(r)
@@ -276,14 +276,16 @@ def check_batch(b, O, coords, edges, terrains, nb, formats, per_chunk=False):
 # take every chunk ticket in turn; staged; pipelined with one workgroup per
 # chunk, with 3 workgroups (two fill, one meshes from the start), with 4
 # workgroups of which two mesh from the start, and with a single workgroup
-# (a filler: it fills every chunk, then meshes them all)
+# (a filler: it fills every chunk, then meshes them all); the staged kernels
+# split over two streams
 MODES = [pytest.param((V.BATCH_FUSED, 0), id="fused"),
          pytest.param((V.BATCH_FUSED, 3), id="fused_persistent3"),
          pytest.param((V.BATCH_STAGED, 0), id="staged"),
          pytest.param((V.BATCH_PIPELINED, 0), id="pipelined"),
          pytest.param((V.BATCH_PIPELINED, 3), id="pipelined_persistent3"),
          pytest.param((V.BATCH_PIPELINED, 4, 2), id="pipelined_persistent4_roles2"),
-         pytest.param((V.BATCH_PIPELINED, 1), id="pipelined_single")]
+         pytest.param((V.BATCH_PIPELINED, 1), id="pipelined_single"),
+         pytest.param((V.BATCH_SPLIT, 0), id="split")]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -553,6 +555,21 @@ def test_batch_ragged_autotune_keeps_results(ctx, oracle_mod):
     b.close()
 
 
+def test_batch_split_with_counting_fills(ctx, oracle_mod):
+    """The split form on 2,048 chunks of 16^3 with no neighbour inside the
+    batch (the fills count; each half has 1,024 whole-chunk fills) and on
+    a 64-chunk grid whose first half borders the second: ids and meshes
+    byte for byte."""
+    say("batch split")
+    coords = [(x, 0, z) for x in range(32) for z in range(64)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [16] * len(coords), [C2] * len(coords),
+                    np.full((len(coords), 4), -1), formats=V.FMT_VERTICES,
+                    mode=(V.BATCH_SPLIT, 0))
+    coords = [(x, 0, z) for x in range(8) for z in range(8)]
+    batch_vs_oracle(ctx, oracle_mod, coords, [32] * 64, [C2] * 64, V.grid_neighbours(8, 8),
+                    mode=(V.BATCH_SPLIT, 0))
+
+
 def test_batch_fill_counts_against_resident_neighbours(ctx, oracle_mod):
     """1,024 chunks of 16^3 (one fill workgroup each) with no neighbour
     inside the batch: the staged fill counts the faces itself, so the batch
@@ -577,6 +594,10 @@ def test_batch_fill_counts_against_resident_neighbours(ctx, oracle_mod):
         if x == 1:
             nb[i, 0] = -2 - z
     b = V.Batch(ctx, coords, 16, tid, nb, formats=V.FMT_VERTICES | V.FMT_INSTANCES, ext=ext)
+    # halves of 512 chunks cannot have whole-chunk fills: the split form is refused
+    with pytest.raises(V.VoxelError) as e:
+        b.set_mode(V.BATCH_SPLIT)
+    assert e.value.status == V.ERR_INVALID
     b.set_mode(V.BATCH_STAGED)
     ot = oterrain(O, t)
     ids = {c: O.chunk_fill(ot, *c, 16) for c in coords}

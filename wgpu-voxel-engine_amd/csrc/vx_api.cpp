@@ -382,6 +382,7 @@ extern "C" int vx_ctx_destroy(vx_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->free_fence) (void)hipEventDestroy(ctx->free_fence);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;  // DeviceHeap frees every device block
     return VX_OK;

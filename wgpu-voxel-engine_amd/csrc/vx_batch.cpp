@@ -130,6 +130,8 @@ bool check_schedules() {
     return on;
 }
 
+int form_of(const vx_batch *b);
+
 bool schedules_ok(vx_batch *b) {
     std::vector<int32_t> h((size_t)std::max(b->n, b->n_items));
     auto fetch = [&](const int32_t *a, int32_t m) {
@@ -139,6 +141,9 @@ bool schedules_ok(vx_batch *b) {
     };
     if (b->order_dev && !(fetch(b->order_dev, b->n) && is_schedule(h.data(), b->n))) return false;
     if (!fetch(b->emit_order, b->n_items)) return false;
+    if (form_of(b) == VX_BATCH_SPLIT)  // each half's order, indices relative to the half
+        return is_schedule(h.data(), b->split_m) &&
+               is_schedule(h.data() + b->split_m, b->n - b->split_m);
     if (!b->slabs) return is_schedule(h.data(), b->n);
     std::vector<int32_t> per(b->n);
     for (int i = 0; i < b->n; i++) per[i] = b->reqs[i].edge == 64 ? kSlabs : 1;
@@ -180,18 +185,22 @@ hipError_t enqueue_emit(vx_batch *b) {
 // graph at every size, from 64 chunks (0.20 vs 0.21 ms) through a 512-chunk
 // frame with its cull job (0.23 vs 0.24 ms) to 4096 chunks (0.76 vs 1.1 ms).
 int form_of(const vx_batch *b) {
-    if (!b->fused_ok || b->mode == VX_BATCH_STAGED) return VX_BATCH_STAGED;
     int m = b->mode;
     if (m == VX_BATCH_AUTO) {
         if (!b->tuned) return VX_BATCH_STAGED;
         m = b->tuned_mode;
     }
+    if (m == VX_BATCH_SPLIT) return b->split_ok ? VX_BATCH_SPLIT : VX_BATCH_STAGED;
+    if (!b->fused_ok) return VX_BATCH_STAGED;
     if (m == VX_BATCH_PIPELINED && !b->pipe_ok) return VX_BATCH_STAGED;
     return m == VX_BATCH_FUSED || m == VX_BATCH_PIPELINED ? m : VX_BATCH_STAGED;
 }
 
 // A persistent form (one launch per run, the fused or the pipelined kernel).
-bool use_fused(const vx_batch *b) { return form_of(b) != VX_BATCH_STAGED; }
+bool use_fused(const vx_batch *b) {
+    const int f = form_of(b);
+    return f == VX_BATCH_FUSED || f == VX_BATCH_PIPELINED;
+}
 
 int32_t fused_grid_of(const vx_batch *b) {
     return b->mode == VX_BATCH_AUTO && b->tuned ? b->tuned_grid : b->grid;
@@ -213,8 +222,61 @@ hipError_t enqueue_fused(vx_batch *b) {
                         ctx->stream);
 }
 
+// The split form: the staged kernels over two halves on two streams.  The
+// context stream fills chunks [0, m2) (the first half [0, m) and the later
+// chunks its counts read planes of) and then [m2, n) back to back, so the
+// fills stay on one queue; the second stream counts, scans and emits the
+// first half as soon as its fill is out, beside the second fill.  The
+// context stream then counts [m, n), scans it from the first half's total
+// and emits it, and waits for the second stream before the run ends.  Each
+// half's emission order holds indices relative to the half.
+hipError_t enqueue_split(vx_batch *b) {
+    vx_ctx *ctx = b->ctx;
+    hipError_t e = hipSuccess;
+    if (!ctx->stream2) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    hipStream_t s0 = ctx->stream, s1 = ctx->stream2;
+    const int32_t n = b->n, m = b->split_m, m2 = b->split_m2;
+    const int32_t fs = fill_split_of(b), es = emit_split_of(b);
+    e = launch_cull(b->cull, s0);
+    if (e == hipSuccess)
+        e = launch_fill(b->fill_dev, nullptr, m2, b->max_edge, ctx->terrains_dev, ctx->lut_dev,
+                        b->overflow + 1, fs, s0);
+    if (e == hipSuccess) e = hipEventRecord(b->split_ev[0], s0);
+    // first half on the second stream
+    if (e == hipSuccess) e = hipStreamWaitEvent(s1, b->split_ev[0], 0);
+    if (e == hipSuccess && !b->fill_counts)
+        e = launch_count(b->mesh_dev, m, b->max_edge, b->counts, nullptr, s1);
+    if (e == hipSuccess)
+        e = launch_scan(b->counts, m, nullptr, b->bases, b->split_total, b->overflow,
+                        b->emit_order, s1);
+    if (e == hipSuccess) e = hipEventRecord(b->split_ev[1], s1);
+    if (e == hipSuccess)
+        e = launch_emit(b->mesh_dev, b->emit_order, m, m, b->max_edge, b->bases, b->counts,
+                        nullptr, b->out, ctx->tables_dev, b->overflow, es, s1);
+    if (e == hipSuccess) e = hipEventRecord(b->split_ev[2], s1);
+    // second half on the context stream
+    if (e == hipSuccess)
+        e = launch_fill(b->fill_dev + m2, nullptr, n - m2, b->max_edge, ctx->terrains_dev,
+                        ctx->lut_dev, b->overflow + 1, fs, s0);
+    if (e == hipSuccess && !b->fill_counts)
+        e = launch_count(b->mesh_dev + m, n - m, b->max_edge, b->counts + m, nullptr, s0);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->split_ev[1], 0);
+    // the overflow word was cleared by the first half's scan: not again here
+    if (e == hipSuccess)
+        e = launch_scan(b->counts + m, n - m, nullptr, b->bases + m, b->total, nullptr,
+                        b->emit_order + m, s0, b->split_total);
+    if (e == hipSuccess)
+        e = launch_emit(b->mesh_dev + m, b->emit_order + m, n - m, n - m, b->max_edge,
+                        b->bases + m, b->counts + m, nullptr, b->out, ctx->tables_dev,
+                        b->overflow, es, s0);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s0, b->split_ev[2], 0);
+    return e;
+}
+
 hipError_t enqueue_all(vx_batch *b) {
     if (use_fused(b)) return enqueue_fused(b);
+    if (form_of(b) == VX_BATCH_SPLIT) return enqueue_split(b);
     hipError_t e = launch_cull(b->cull, b->ctx->stream);
     if (e == hipSuccess) e = enqueue_fill(b);
     if (e == hipSuccess && !b->fill_counts) e = enqueue_count(b);
@@ -258,6 +320,8 @@ void free_batch(vx_batch *b) {
     ctx->dev_free(b->arena);
     ctx->dev_free(b->fused_dev);
     ctx->dev_free(b->fctl);
+    for (hipEvent_t &ev : b->split_ev)
+        if (ev) (void)hipEventDestroy(ev);
     delete b;
 }
 
@@ -318,7 +382,8 @@ void tune_load() {
         const int k = sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &a, &c, &d, &e, &h, &m, &g, &r,
                              &fs, &es);
         if (k != 7 && k != 8 && k != 10) continue;
-        const bool mode_ok = m == VX_BATCH_STAGED || m == VX_BATCH_FUSED || m == VX_BATCH_PIPELINED;
+        const bool mode_ok = m == VX_BATCH_STAGED || m == VX_BATCH_FUSED ||
+                             m == VX_BATCH_PIPELINED || m == VX_BATCH_SPLIT;
         const bool ratio_ok = r == 0 || (r >= 2 && r <= kPipeRatioMax);
         if (mode_ok && ratio_ok && g >= 0 && fill_split_ok(fs) && emit_split_ok(es))
             g_tune[TuneKey{a, c, d, e, h}] = TuneChoice{m, g, r, fs, es};
@@ -480,6 +545,24 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
             fw[i].count_out = b->counts + i;
             fw[i].slab_out = b->slabs ? b->slabs + (size_t)kSlabs * i : nullptr;
         }
+    // split form (edges up to 32): the first half [0, m) and the later chunks
+    // it borders, up to m2, are filled first; a batch whose fill counts needs
+    // whole-chunk fills (kFillGrid chunks or more) in both fill launches
+    if (b->max_edge <= 32 && n >= 2) {
+        const int32_t m = n / 2;
+        int32_t m2 = m;
+        for (int i = 0; i < m; i++)
+            for (int k = 0; k < 4; k++) m2 = std::max(m2, reqs[i].nb[k] + 1);
+        b->split_m = m;
+        b->split_m2 = m2;
+        b->split_ok = !b->fill_counts || (m2 >= kFillGrid && n - m2 >= kFillGrid);
+        b->split_total = (uint64_t *)((char *)b->total + 16);  // in the control block's spare words
+        for (int k = 0; k < 3 && b->split_ok; k++)
+            if (hipEventCreateWithFlags(&b->split_ev[k], hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                b->split_ok = false;
+            }
+    }
     // same terrain, so its facing plane can be recomputed from noise; the
     // reference-rule terrain is generated by the fill kernels only
     static const int ddx[4] = {-1, 1, 0, 0}, ddz[4] = {0, 0, -1, 1};
@@ -575,7 +658,7 @@ extern "C" int vx_batch_create_ext(vx_ctx *ctx, const vx_batch_req *reqs, int32_
         auto it = g_tune.find(tune_key(b));
         const int m = it != g_tune.end() ? it->second.mode : VX_BATCH_STAGED;
         if (it != g_tune.end() && (m != VX_BATCH_FUSED || b->fused_ok) &&
-            (m != VX_BATCH_PIPELINED || b->pipe_ok)) {
+            (m != VX_BATCH_PIPELINED || b->pipe_ok) && (m != VX_BATCH_SPLIT || b->split_ok)) {
             b->tuned = true;
             b->tuned_mode = it->second.mode;
             b->tuned_grid = it->second.grid;
@@ -597,10 +680,11 @@ extern "C" int vx_batch_destroy(vx_batch *b) {
 }
 
 extern "C" int vx_batch_set_mode(vx_batch *b, int32_t mode) {
-    if (!b || mode < VX_BATCH_AUTO || mode > VX_BATCH_PIPELINED) return VX_ERR_INVALID;
+    if (!b || mode < VX_BATCH_AUTO || mode > VX_BATCH_SPLIT) return VX_ERR_INVALID;
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     if (mode == VX_BATCH_FUSED && !b->fused_ok) return VX_ERR_INVALID;
     if (mode == VX_BATCH_PIPELINED && !b->pipe_ok) return VX_ERR_INVALID;
+    if (mode == VX_BATCH_SPLIT && !b->split_ok) return VX_ERR_INVALID;
     if (mode != b->mode) {
         (void)hipStreamSynchronize(b->ctx->stream);
         drop_graph(b);
@@ -929,6 +1013,7 @@ extern "C" int vx_batch_autotune(vx_ctx *ctx, vx_batch *b, int32_t reps, vx_tune
         for (int per = 3; per <= 4; per++)
             if (cus * per < b->n) cands.push_back({VX_BATCH_FUSED, cus * per, 0});
     }
+    if (b->split_ok) cands.push_back({VX_BATCH_SPLIT, 0, 0});
     if (b->pipe_ok) {
         // fill and mesh roles: one workgroup per chunk with one mesher in
         // three; 8 per CU with one mesher in two, three or four
@@ -1022,9 +1107,11 @@ extern "C" int vx_batch_profile(vx_ctx *ctx, vx_batch *b, int32_t reps, float ms
     for (int i = 0; i < 6; i++) VX_HIP(ctx, hipEventCreate(&ev[i]));
     double acc[5] = {0, 0, 0, 0, 0};
     int st = VX_OK;
-    for (int r = 0; r < reps && !st && use_fused(b); r++) {
+    // one-launch forms and the split form: the whole run as one figure
+    const bool whole = use_fused(b) || form_of(b) == VX_BATCH_SPLIT;
+    for (int r = 0; r < reps && !st && whole; r++) {
         hipError_t e = hipEventRecord(ev[0], ctx->stream);
-        if (e == hipSuccess) e = enqueue_fused(b);
+        if (e == hipSuccess) e = enqueue_all(b);
         if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
         if (e != hipSuccess) {
@@ -1036,7 +1123,7 @@ extern "C" int vx_batch_profile(vx_ctx *ctx, vx_batch *b, int32_t reps, float ms
         (void)hipEventElapsedTime(&t, ev[0], ev[1]);
         acc[0] += t;
     }
-    for (int r = 0; r < reps && !st && !use_fused(b); r++) {
+    for (int r = 0; r < reps && !st && !whole; r++) {
         hipError_t e = hipEventRecord(ev[0], ctx->stream);
         if (e == hipSuccess) e = enqueue_fill(b);
         if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);

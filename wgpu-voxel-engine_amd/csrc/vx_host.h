@@ -69,6 +69,7 @@ inline bool coord_in_range(vx_coord c, int32_t edge) {
 struct vx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // second branch of split-batch runs (made on first use)
     std::recursive_mutex mu;
     uint64_t budget = 0;
     std::string last_error;
@@ -163,6 +164,12 @@ struct vx_batch {
     bool fused_ok = false;          // fused form eligible
     bool pipe_ok = false;           // pipelined form eligible (fused_ok, edges <= 32)
     bool fill_counts = false;       // the staged fill also counts (no count kernel)
+    bool split_ok = false;          // VX_BATCH_SPLIT eligible
+    int32_t split_m = 0;            // split form: chunks [0, split_m) are the first half
+    int32_t split_m2 = 0;           // its first fill covers [0, split_m2) (the half and the
+                                    // later chunks it borders)
+    uint64_t *split_total = nullptr;   // faces of the first half (device)
+    hipEvent_t split_ev[3] = {};    // fill, scan of the first half; the second half's end
     int32_t pipe_lag = 0;           // farthest in-batch neighbour index ahead of a chunk
     vx::FusedWork *fused_dev = nullptr;
     void *fctl = nullptr;           // ticket | error | status[n]

@@ -202,8 +202,11 @@ inline bool fill_whole_chunks(int32_t n) { return n >= kFillGrid; }
 constexpr int kSlabs = 4;
 hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint32_t *counts,
                         uint32_t *slabs, hipStream_t s);
+// base_in (device, may be null): faces before these chunks, added to every
+// base and to the total (the second half of a split batch).
 hipError_t launch_scan(const uint32_t *counts, int32_t n, const MeshWork *items, uint64_t *bases,
-                       uint64_t *total, uint32_t *overflow, int32_t *order_out, hipStream_t s);
+                       uint64_t *total, uint32_t *overflow, int32_t *order_out, hipStream_t s,
+                       const uint64_t *base_in = nullptr);
 hipError_t launch_fill_one(const FillWork &w, const TerrainDev *terrains, const GradLut *lut,
                            hipStream_t s);
 hipError_t launch_pack_one(const FillWork &w, hipStream_t s);

@@ -1136,7 +1136,8 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
                                                     uint64_t *__restrict__ bases,
                                                     uint64_t *__restrict__ total,
                                                     uint32_t *__restrict__ overflow,
-                                                    int32_t *__restrict__ order_out) {
+                                                    int32_t *__restrict__ order_out,
+                                                    const uint64_t *__restrict__ base_in) {
     __shared__ uint64_t s_wave[16];
     __shared__ uint32_t s_hist[kOrderBuckets];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1173,7 +1174,8 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
         if (i < wave) wbase += s_wave[i];
         sum += s_wave[i];
     }
-    uint64_t run = wbase + incl - local;
+    const uint64_t off = base_in ? *base_in : 0ull;  // faces before these chunks
+    uint64_t run = off + wbase + incl - local;
 #pragma unroll
     for (int j = 0; j < kCached; j++) {
         if (b + j < ec) bases[b + j] = run;
@@ -1184,7 +1186,7 @@ __global__ void __launch_bounds__(1024) scan_kernel(const uint32_t *__restrict__
         run += counts[i];
     }
     if (tid == 0) {
-        *total = sum;
+        *total = off + sum;
         if (overflow) *overflow = 0u;  // the emit stage that follows sets it (not overflow[1])
     }
     if (!order_out) return;
@@ -2158,9 +2160,10 @@ hipError_t launch_count(const MeshWork *work, int32_t n, int32_t max_edge, uint3
 }
 
 hipError_t launch_scan(const uint32_t *counts, int32_t n, const MeshWork *items, uint64_t *bases,
-                       uint64_t *total, uint32_t *overflow, int32_t *order_out, hipStream_t s) {
+                       uint64_t *total, uint32_t *overflow, int32_t *order_out, hipStream_t s,
+                       const uint64_t *base_in) {
     if (n <= 0) return hipSuccess;
-    scan_kernel<<<1, 1024, 0, s>>>(counts, n, items, bases, total, overflow, order_out);
+    scan_kernel<<<1, 1024, 0, s>>>(counts, n, items, bases, total, overflow, order_out, base_in);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DEVICE, ERR_CAPACITY, ERR_BUDGET, ERR_NOT_READY,
 TERRAIN_AIR, TERRAIN_SOLID, TERRAIN_HEIGHTMAP, TERRAIN_DENSITY, TERRAIN_REFERENCE = range(5)
 FMT_INSTANCES, FMT_VERTICES = 1, 2
 FMT_INDICES = 4  # vx_batch_read_mesh selector for the index buffer
-BATCH_AUTO, BATCH_FUSED, BATCH_STAGED, BATCH_PIPELINED = 0, 1, 2, 3
+BATCH_AUTO, BATCH_FUSED, BATCH_STAGED, BATCH_PIPELINED, BATCH_SPLIT = 0, 1, 2, 3, 4
 AIR, GRASS, DIRT = 0, 1, 2
 
 INSTANCE_DTYPE = np.dtype([("m", "<f4", (4, 4)), ("tex", "<u4")])  # InstanceRaw, 68 B
@@ -107,11 +107,11 @@ class SimResult(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("mode", C.c_int32), ("grid", C.c_int32), ("best_ms", C.c_float),
-                ("n_candidates", C.c_int32), ("cand_mode", C.c_int32 * 8),
-                ("cand_grid", C.c_int32 * 8), ("cand_ms", C.c_float * 8),
-                ("ratio", C.c_int32), ("cand_ratio", C.c_int32 * 8),
+                ("n_candidates", C.c_int32), ("cand_mode", C.c_int32 * 12),
+                ("cand_grid", C.c_int32 * 12), ("cand_ms", C.c_float * 12),
+                ("ratio", C.c_int32), ("cand_ratio", C.c_int32 * 12),
                 ("fill_split", C.c_int32), ("emit_split", C.c_int32),
-                ("cand_fill_split", C.c_int32 * 8), ("cand_emit_split", C.c_int32 * 8)]
+                ("cand_fill_split", C.c_int32 * 12), ("cand_emit_split", C.c_int32 * 12)]
 
 
 class WorldDesc(C.Structure):
@@ -650,7 +650,8 @@ class Batch:
         r = TuneResult()
         self.ctx.check(self.ctx.L.vx_batch_autotune(self.ctx.h, self.h, reps, C.byref(r)),
                        "vx_batch_autotune")
-        names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged", BATCH_PIPELINED: "pipelined"}
+        names = {BATCH_FUSED: "fused", BATCH_STAGED: "staged", BATCH_PIPELINED: "pipelined",
+                 BATCH_SPLIT: "split"}
         return {"mode": names.get(r.mode, r.mode), "grid": r.grid, "ratio": r.ratio,
                 "splits": [r.fill_split, r.emit_split], "best_ms": round(r.best_ms, 4),
                 "candidates": [{"mode": names.get(r.cand_mode[i], r.cand_mode[i]),
