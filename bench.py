@@ -533,11 +533,12 @@ def main():
 
     # every other form, for comparison (same outputs): timed like `value`,
     # kernels timed with HIP events
-    names_of = {V.BATCH_STAGED: "staged", V.BATCH_FUSED: "fused", V.BATCH_PIPELINED: "pipelined"}
+    names_of = {V.BATCH_STAGED: "staged", V.BATCH_FUSED: "fused", V.BATCH_PIPELINED: "pipelined",
+                V.BATCH_SPLIT: "split"}
     form_ms = {mode: chosen_ms}
     other = []
     reps2 = max(5, args.steps // 2)
-    for m in (V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED):
+    for m in (V.BATCH_STAGED, V.BATCH_FUSED, V.BATCH_PIPELINED, V.BATCH_SPLIT):
         if m == mode:
             continue
         b.set_mode(m)
